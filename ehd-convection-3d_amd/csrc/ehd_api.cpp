@@ -1,0 +1,731 @@
+// ehd_api.cpp -- the C-ABI (include/ehd.h): box lifetime, parameter
+// derivation, initial states, the coupled step loop, field I/O and
+// diagnostics.  Host C++ over the HIP runtime; one HIP stream per box.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "ehd_internal.h"
+#include "fft_plan.h"
+
+using namespace ehd;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(EHD_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr double kPi = 3.14159265358979323846;
+
+}  // namespace
+
+struct ehd_box {
+    ehd_params prm;
+    ehd_derived der;
+    LbmParams lp;
+    Prec prec;
+    int nx, ny, nz, nxh;
+    size_t S, nxy, esz;
+    hipStream_t st = nullptr;
+    void* f[2] = {nullptr, nullptr};
+    void* f0[2] = {nullptr, nullptr};
+    void* h[2] = {nullptr, nullptr};
+    void* h0[2] = {nullptr, nullptr};
+    int cur = 0;
+    bool direct = true;        // state holds pre-collision populations
+    bool ebuf = true;          // next collide takes E from ex/ey/ez
+    bool initialised = false;
+    void *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *q = nullptr, *phi = nullptr;
+    double* phibar = nullptr;
+    double *ex = nullptr, *ey = nullptr, *ez = nullptr;
+    void* spec = nullptr;
+    double* dscr = nullptr;
+    double* cpt = nullptr;
+    void* twx = nullptr;
+    void* twy = nullptr;
+    double* diag_scr = nullptr;
+    double* diag_out = nullptr;
+    long long step = 0;
+    double t = 0.0;
+    double current_hs = 0.0;
+    // profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> lbm_pairs, poi_pairs, step_pairs;
+    double lbm_ms = 0.0, poi_ms = 0.0, step_ms = 0.0;
+    long long lbm_n = 0, poi_n = 0, step_n = 0;
+};
+
+namespace {
+
+hipEvent_t take_event(ehd_box* b)
+{
+    if (!b->ev_pool.empty()) {
+        hipEvent_t e = b->ev_pool.back();
+        b->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int resolve_events(ehd_box* b)
+{
+    if (b->lbm_pairs.empty() && b->poi_pairs.empty() && b->step_pairs.empty()) return EHD_OK;
+    HIPCHK(hipStreamSynchronize(b->st));
+    auto drain = [&](std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, double& acc, long long& n) -> int {
+        for (auto& pr : v) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+            acc += ms;
+            n += 1;
+            b->ev_pool.push_back(pr.first);
+            b->ev_pool.push_back(pr.second);
+        }
+        v.clear();
+        return EHD_OK;
+    };
+    int rc;
+    if ((rc = drain(b->lbm_pairs, b->lbm_ms, b->lbm_n)) != EHD_OK) return rc;
+    if ((rc = drain(b->poi_pairs, b->poi_ms, b->poi_n)) != EHD_OK) return rc;
+    if ((rc = drain(b->step_pairs, b->step_ms, b->step_n)) != EHD_OK) return rc;
+    return EHD_OK;
+}
+
+int validate(const ehd_params* p)
+{
+    if (!p) return fail(EHD_EINVAL, "null params");
+    if (p->nx < 2 || p->ny < 2 || (p->nx & 1) || (p->ny & 1))
+        return fail(EHD_EINVAL, "nx and ny must be even and >= 2 (got %d, %d)", p->nx, p->ny);
+    if (p->nz < 4) return fail(EHD_EINVAL, "nz must be >= 4 (got %d)", p->nz);
+    const double S = (double)p->nx * p->ny * p->nz;
+    if (S * 27.0 >= 4294967295.0) return fail(EHD_EINVAL, "grid too large: 27*nx*ny*nz must fit 32 bits");
+    if (!(p->T > 0) || !(p->C > 0) || !(p->M > 0) || !(p->Fe > 0))
+        return fail(EHD_EINVAL, "T, C, M, Fe must be positive");
+    if (!(p->cfl > 0) || !(p->voltage > 0) || !(p->eps > 0) || !(p->rho0 > 0) || !(p->lz > 0))
+        return fail(EHD_EINVAL, "cfl, voltage, eps, rho0, lz must be positive");
+    if (!(p->lambda_f > 0) || !(p->lambda_c > 0)) return fail(EHD_EINVAL, "TRT magic parameters must be positive");
+    if (p->precision != EHD_FP64 && p->precision != EHD_FP32S) return fail(EHD_EINVAL, "unknown precision");
+    if (p->init_mode < EHD_INIT_REFERENCE || p->init_mode > EHD_INIT_FIELDS) return fail(EHD_EINVAL, "unknown init mode");
+    return EHD_OK;
+}
+
+void fill_lbm_params(ehd_box* b)
+{
+    const ehd_params& p = b->prm;
+    const ehd_derived& d = b->der;
+    LbmParams& L = b->lp;
+    L.nx = p.nx; L.ny = p.ny; L.nz = p.nz;
+    L.nxy = (unsigned)(p.nx * p.ny);
+    L.S = (unsigned)((size_t)p.nx * p.ny * p.nz);
+    L.dt = d.dt; L.cfl = p.cfl; L.cs_square = d.cs_square; L.K = d.K; L.exf = p.exf;
+    L.charge0 = d.charge0; L.rho0 = p.rho0;
+    L.w[0] = 8.0 / 27.0; L.w[1] = 2.0 / 27.0; L.w[2] = 1.0 / 54.0; L.w[3] = 1.0 / 216.0;
+    // LBM.cu:299-302
+    const double nu = d.nu, cs_square = d.cs_square, dt = d.dt, diffu = d.diffu;
+    L.omega_plus = 1.0 / (nu / cs_square / dt + 1.0 / 2.0) / dt;
+    L.omega_minus = 1.0 / (p.lambda_f / (nu / cs_square / dt) + 1.0 / 2.0) / dt;
+    L.omega_c_minus = 1.0 / (diffu / cs_square / dt + 1.0 / 2.0) / dt;
+    L.omega_c_plus = 1.0 / (p.lambda_c / (diffu / cs_square / dt) + 1.0 / 2.0) / dt;
+    // LBM.cu:1137-1139 and 1288-1291
+    for (int c = 0; c < 4; ++c) {
+        L.multi[c] = 2.0 * p.rho0 * p.uw / cs_square * L.w[c] / p.cfl;
+        L.inj[c] = 2.0 * d.charge0 * L.w[c];
+    }
+    L.dx = d.dx; L.dy = d.dy; L.dz = d.dz;
+    L.q_ref = d.charge0;
+}
+
+template <typename T>
+void make_twiddles(int n, std::vector<T>& out)
+{
+    out.resize(2 * (size_t)n);
+    for (int j = 0; j < n; ++j) {
+        const double a = 2.0 * kPi * (double)j / (double)n;
+        out[2 * j] = (T)std::cos(a);
+        out[2 * j + 1] = (T)(-std::sin(a));
+    }
+}
+
+int alloc(void** p, size_t bytes)
+{
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        return fail(EHD_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+    }
+    return EHD_OK;
+}
+
+void free_box(ehd_box* b)
+{
+    void* ptrs[] = {b->f[0], b->f[1], b->f0[0], b->f0[1], b->h[0], b->h[1], b->h0[0], b->h0[1],
+                    b->rho, b->ux, b->uy, b->uz, b->q, b->phi, b->phibar, b->ex, b->ey, b->ez,
+                    b->spec, b->dscr, b->cpt, b->twx, b->twy, b->diag_scr, b->diag_out};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto& pr : b->lbm_pairs) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    for (auto& pr : b->poi_pairs) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    for (auto& pr : b->step_pairs) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    for (hipEvent_t e : b->ev_pool) (void)hipEventDestroy(e);
+    if (b->st) (void)hipStreamDestroy(b->st);
+}
+
+PopBufs pop_bufs(ehd_box* b)
+{
+    const int A = b->cur, B = 1 - b->cur;
+    return PopBufs{b->f[A], b->f0[A], b->h[A], b->h0[A], b->f[B], b->f0[B], b->h[B], b->h0[B]};
+}
+
+MacroBufs macro_bufs(ehd_box* b)
+{
+    return MacroBufs{b->rho, b->ux, b->uy, b->uz, b->q, b->phi, b->phibar, b->ex, b->ey, b->ez};
+}
+
+int run_lbm(ehd_box* b, bool macro)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (b->prof) {
+        e0 = take_event(b);
+        e1 = take_event(b);
+        if (!e0 || !e1) return fail(EHD_EHIP, "hipEventCreate failed");
+        HIPCHK(hipEventRecord(e0, b->st));
+    }
+    hipError_t e;
+    if (b->prec == Prec::F64) e = launch_lbm_f64(b->lp, pop_bufs(b), macro_bufs(b), b->direct, b->ebuf, macro, b->st);
+    else e = launch_lbm_f32s(b->lp, pop_bufs(b), macro_bufs(b), b->direct, b->ebuf, macro, b->st);
+    if (e != hipSuccess) return fail(EHD_EHIP, "LBM launch failed: %s", hipGetErrorString(e));
+    if (b->prof) {
+        HIPCHK(hipEventRecord(e1, b->st));
+        b->lbm_pairs.emplace_back(e0, e1);
+    }
+    b->cur = 1 - b->cur;
+    b->direct = false;
+    return EHD_OK;
+}
+
+int run_poisson(ehd_box* b)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (b->prof) {
+        e0 = take_event(b);
+        e1 = take_event(b);
+        if (!e0 || !e1) return fail(EHD_EHIP, "hipEventCreate failed");
+        HIPCHK(hipEventRecord(e0, b->st));
+    }
+    PoissonBufs pb{b->nx, b->ny, b->nz, b->nxh, b->spec, b->dscr, b->cpt, b->twx, b->twy, b->phibar};
+    hipError_t e = launch_poisson(b->prec, b->lp, pb, b->q, b->phi, b->prm.voltage, b->prm.eps, b->st);
+    if (e != hipSuccess) return fail(EHD_EHIP, "Poisson launch failed: %s", hipGetErrorString(e));
+    if (b->prof) {
+        HIPCHK(hipEventRecord(e1, b->st));
+        b->poi_pairs.emplace_back(e0, e1);
+    }
+    b->ebuf = false;
+    return EHD_OK;
+}
+
+// Hydrostatic unipolar-injection profile (D = 0, u = 0), nondimensional:
+// E^2 = E0^2 + 2 C E0 zeta, q = C E0/E, phi = 1 - (E^3 - E0^3)/(3 C E0),
+// with E0 fixed by phi(1) = 0 (bisection to the last bit).
+double hydrostatic_e0(double C)
+{
+    double lo = 0.0, hi = 1.0;
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        const double e1 = std::sqrt(mid * mid + 2.0 * C * mid);
+        const double g = (e1 * e1 * e1 - mid * mid * mid) / (3.0 * C * mid) - 1.0;
+        if (g > 0.0) hi = mid;
+        else lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ehd_abi_version(void) { return EHD_ABI_VERSION; }
+const char* ehd_last_error(void) { return g_err; }
+
+int ehd_params_default(ehd_params* p)
+{
+    if (!p) return fail(EHD_EINVAL, "null params");
+    std::memset(p, 0, sizeof(*p));
+    p->nx = 122; p->ny = 122; p->nz = 101;                     // LBM.h:34-36
+    const double K = 2.5e-5, eps = 1.0e-4, V = 1.0e4, nu = 0.0251004, rho0 = 1600.0, q0 = 10.0,
+                 D = 6.25e-5, Lz = 1.0;                        // LBM.h:54-62
+    p->M = std::sqrt(eps / rho0) / K;                          // LBM.cu:1375-1378
+    p->T = eps * V / K / nu / rho0;
+    p->C = q0 * Lz * Lz / (V * eps);
+    p->Fe = K * V / D;
+    p->Pr = 10.0;
+    p->lambda_f = 1.0 / 12.0; p->lambda_c = 1.0e-6;            // LBM.h:79-80
+    p->cfl = 0.01; p->voltage = V; p->eps = eps; p->rho0 = rho0; p->lz = Lz;
+    p->uw = 0.0; p->exf = 0.0;
+    p->precision = EHD_FP64;
+    p->init_mode = EHD_INIT_REFERENCE;
+    p->seed = 1;
+    p->perturb_amp = 1.0e-3;
+    return EHD_OK;
+}
+
+int ehd_derive(const ehd_params* p, ehd_derived* d)
+{
+    int rc = validate(p);
+    if (rc) return rc;
+    if (!d) return fail(EHD_EINVAL, "null output");
+    std::memset(d, 0, sizeof(*d));
+    d->dz = p->lz / (double)(p->nz - 1);
+    d->dx = d->dz;
+    d->dy = d->dz;
+    d->lx = p->nx * d->dx;
+    d->ly = p->ny * d->dy;
+    d->dt = p->cfl * d->dx;                       // LBM.h:52
+    d->cs_square = 1.0 / 3.0 / (p->cfl * p->cfl);  // LBM.h:53
+    d->K = std::sqrt(p->eps / p->rho0) / p->M;
+    d->nu = p->eps * p->voltage / (d->K * p->rho0 * p->T);
+    d->charge0 = p->C * p->voltage * p->eps / (p->lz * p->lz);
+    d->diffu = d->K * p->voltage / p->Fe;
+    const double tp = d->nu / d->cs_square / d->dt;      // tau+ - 1/2
+    const double tcm = d->diffu / d->cs_square / d->dt;  // tau_c- - 1/2
+    d->tau_plus = tp + 0.5;
+    d->tau_minus = p->lambda_f / tp + 0.5;
+    d->tau_c_minus = tcm + 0.5;
+    d->tau_c_plus = p->lambda_c / tcm + 0.5;
+    return EHD_OK;
+}
+
+int ehd_compute_parameters(const ehd_derived* d, const ehd_params* p, double* T, double* M, double* C, double* Fe)
+{
+    if (!d || !p || !T || !M || !C || !Fe) return fail(EHD_EINVAL, "null argument");
+    *M = std::sqrt(p->eps / p->rho0) / d->K;     // LBM.cu:1375-1378
+    *T = p->eps * p->voltage / d->K / d->nu / p->rho0;
+    *C = d->charge0 * p->lz * p->lz / (p->voltage * p->eps);
+    *Fe = d->K * p->voltage / d->diffu;
+    return EHD_OK;
+}
+
+int ehd_create(const ehd_params* p, ehd_box** out)
+{
+    if (!out) return fail(EHD_EINVAL, "null output");
+    *out = nullptr;
+    int rc = validate(p);
+    if (rc) return rc;
+    ehd_box* b = new (std::nothrow) ehd_box();
+    if (!b) return fail(EHD_ENOMEM, "host allocation failed");
+    b->prm = *p;
+    ehd_derive(p, &b->der);
+    b->prec = (p->precision == EHD_FP64) ? Prec::F64 : Prec::F32S;
+    b->nx = p->nx; b->ny = p->ny; b->nz = p->nz; b->nxh = p->nx / 2 + 1;
+    b->nxy = (size_t)p->nx * p->ny;
+    b->S = b->nxy * p->nz;
+    b->esz = (b->prec == Prec::F64) ? 8 : 4;
+    fill_lbm_params(b);
+
+    auto bail = [&](int code) {
+        free_box(b);
+        delete b;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(EHD_EHIP, "hipStreamCreate failed: %s", hipGetErrorString(hipGetLastError())));
+    const size_t S = b->S, es = b->esz;
+    for (int i = 0; i < 2; ++i) {
+        if ((rc = alloc(&b->f[i], 26 * S * es)) || (rc = alloc(&b->f0[i], S * es)) ||
+            (rc = alloc(&b->h[i], 26 * S * es)) || (rc = alloc(&b->h0[i], S * es)))
+            return bail(rc);
+    }
+    if ((rc = alloc(&b->rho, S * es)) || (rc = alloc(&b->ux, S * es)) || (rc = alloc(&b->uy, S * es)) ||
+        (rc = alloc(&b->uz, S * es)) || (rc = alloc(&b->q, S * es)) || (rc = alloc(&b->phi, S * es)) ||
+        (rc = alloc((void**)&b->phibar, sizeof(double) * b->nz)) || (rc = alloc((void**)&b->ex, sizeof(double) * S)) ||
+        (rc = alloc((void**)&b->ey, sizeof(double) * S)) || (rc = alloc((void**)&b->ez, sizeof(double) * S)))
+        return bail(rc);
+    const size_t nmodes = (size_t)b->nxh * b->ny, nint = (size_t)(b->nz - 2);
+    if ((rc = alloc(&b->spec, nmodes * nint * 2 * es)) || (rc = alloc((void**)&b->dscr, nmodes * nint * 2 * sizeof(double))) ||
+        (rc = alloc((void**)&b->cpt, nmodes * nint * sizeof(double))) || (rc = alloc(&b->twx, 2 * (size_t)b->nx * es)) ||
+        (rc = alloc(&b->twy, 2 * (size_t)b->ny * es)) ||
+        (rc = alloc((void**)&b->diag_scr, sizeof(double) * 8 * (size_t)diag_blocks())) ||
+        (rc = alloc((void**)&b->diag_out, sizeof(double) * 8)))
+        return bail(rc);
+
+    // Thomas pivots of each (kx,ky) mode: b = -(2 + k^2 dz^2), c'_0 = 1/b, c'_k = 1/(b - c'_{k-1}).
+    // Wavenumbers as main.cu:109-125.
+    {
+        std::vector<double> cp(nmodes * nint);
+        const double dz2 = b->der.dz * b->der.dz;
+        for (int ky = 0; ky < b->ny; ++ky) {
+            const double kyv = (ky <= b->ny / 2) ? (double)ky * 2.0 * kPi / b->der.ly
+                                                 : ((double)ky - b->ny) * 2.0 * kPi / b->der.ly;
+            for (int kx = 0; kx < b->nxh; ++kx) {
+                const double kxv = (double)kx * 2.0 * kPi / b->der.lx;
+                const double bd = -(2.0 + (kxv * kxv + kyv * kyv) * dz2);
+                const size_t m = (size_t)ky * b->nxh + kx;
+                double c = 1.0 / bd;
+                cp[m] = c;
+                for (size_t k = 1; k < nint; ++k) {
+                    c = 1.0 / (bd - c);
+                    cp[k * nmodes + m] = c;
+                }
+            }
+        }
+        if (hipMemcpy(b->cpt, cp.data(), cp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(EHD_EHIP, "pivot upload failed"));
+    }
+    if (b->prec == Prec::F64) {
+        std::vector<double> tx, ty;
+        make_twiddles(b->nx, tx);
+        make_twiddles(b->ny, ty);
+        if (hipMemcpy(b->twx, tx.data(), tx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->twy, ty.data(), ty.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(EHD_EHIP, "twiddle upload failed"));
+    } else {
+        std::vector<float> tx, ty;
+        make_twiddles(b->nx, tx);
+        make_twiddles(b->ny, ty);
+        if (hipMemcpy(b->twx, tx.data(), tx.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->twy, ty.data(), ty.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(EHD_EHIP, "twiddle upload failed"));
+    }
+    *out = b;
+    return EHD_OK;
+}
+
+void ehd_destroy(ehd_box* b)
+{
+    if (!b) return;
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    free_box(b);
+    delete b;
+}
+
+int ehd_init(ehd_box* b, const double* fields9)
+{
+    if (!b) return fail(EHD_EINVAL, "null box");
+    const ehd_params& p = b->prm;
+    const size_t S = b->S;
+    if (p.init_mode == EHD_INIT_FIELDS && !fields9) return fail(EHD_EINVAL, "EHD_INIT_FIELDS needs fields");
+    // double macroscopic fields rho,q,phi,u,v,w on the device
+    double* tmp = nullptr;
+    int rc = alloc((void**)&tmp, 6 * S * sizeof(double));
+    if (rc) return rc;
+    double *rho = tmp, *q = tmp + S, *phi = tmp + 2 * S, *ux = tmp + 3 * S, *uy = tmp + 4 * S, *uz = tmp + 5 * S;
+    struct Guard {
+        double* p;
+        ~Guard() { if (p) (void)hipFree(p); }
+    } guard{tmp};
+    double* dqz = nullptr;
+    if (p.init_mode == EHD_INIT_REFERENCE) {
+        HIPCHK(launch_init_fields(b->lp, 0, nullptr, nullptr, 0, 0.0, p.voltage, p.lz, rho, q, phi, ux, uy, uz, b->st));
+        HIPCHK(hipMemsetAsync(b->ex, 0, S * sizeof(double), b->st));
+        HIPCHK(hipMemsetAsync(b->ey, 0, S * sizeof(double), b->st));
+        HIPCHK(hipMemsetAsync(b->ez, 0, S * sizeof(double), b->st));
+    } else if (p.init_mode == EHD_INIT_HYDROSTATIC) {
+        const int nz = b->nz;
+        std::vector<double> tab(2 * (size_t)nz);
+        const double C = b->der.charge0 * p.lz * p.lz / (p.voltage * p.eps);
+        const double e0 = hydrostatic_e0(C);
+        const double qs = p.eps * p.voltage / (p.lz * p.lz);
+        for (int z = 0; z < nz; ++z) {
+            const double zeta = b->der.dz * z / p.lz;
+            const double e = std::sqrt(e0 * e0 + 2.0 * C * e0 * zeta);
+            tab[z] = C * e0 / e * qs;
+            double ph = p.voltage * (1.0 - (e * e * e - e0 * e0 * e0) / (3.0 * C * e0));
+            if (z == 0) ph = p.voltage;
+            if (z == nz - 1) ph = 0.0;
+            tab[nz + z] = ph;
+        }
+        rc = alloc((void**)&dqz, tab.size() * sizeof(double));
+        if (rc) return rc;
+        hipError_t e1 = hipMemcpyAsync(dqz, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, b->st);
+        hipError_t e2 = launch_init_fields(b->lp, 1, dqz, dqz + nz, p.seed, p.perturb_amp, p.voltage, p.lz, rho, q, phi,
+                                           ux, uy, uz, b->st);
+        hipError_t e3 = launch_efield(Prec::F64, b->lp, phi, nullptr, b->ex, b->ey, b->ez, b->st);
+        hipError_t e4 = hipStreamSynchronize(b->st);
+        (void)hipFree(dqz);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess)
+            return fail(EHD_EHIP, "hydrostatic init failed");
+    } else {
+        HIPCHK(hipMemcpyAsync(rho, fields9 + 0 * S, S * 8, hipMemcpyHostToDevice, b->st));
+        HIPCHK(hipMemcpyAsync(q, fields9 + 1 * S, S * 8, hipMemcpyHostToDevice, b->st));
+        HIPCHK(hipMemcpyAsync(phi, fields9 + 2 * S, S * 8, hipMemcpyHostToDevice, b->st));
+        HIPCHK(hipMemcpyAsync(ux, fields9 + 3 * S, S * 8, hipMemcpyHostToDevice, b->st));
+        HIPCHK(hipMemcpyAsync(uy, fields9 + 4 * S, S * 8, hipMemcpyHostToDevice, b->st));
+        HIPCHK(hipMemcpyAsync(uz, fields9 + 5 * S, S * 8, hipMemcpyHostToDevice, b->st));
+        HIPCHK(hipMemcpyAsync(b->ex, fields9 + 6 * S, S * 8, hipMemcpyHostToDevice, b->st));
+        HIPCHK(hipMemcpyAsync(b->ey, fields9 + 7 * S, S * 8, hipMemcpyHostToDevice, b->st));
+        HIPCHK(hipMemcpyAsync(b->ez, fields9 + 8 * S, S * 8, hipMemcpyHostToDevice, b->st));
+    }
+    b->cur = 0;
+    HIPCHK(launch_init_equilibrium(b->prec, b->lp, rho, q, ux, uy, uz, b->ex, b->ey, b->ez, b->f[0], b->f0[0], b->h[0],
+                                   b->h0[0], b->st));
+    // the frozen wall rest populations must be present in both buffers
+    HIPCHK(hipMemcpyAsync(b->f0[1], b->f0[0], S * b->esz, hipMemcpyDeviceToDevice, b->st));
+    HIPCHK(launch_from_double(b->prec, rho, b->rho, S, b->st));
+    HIPCHK(launch_from_double(b->prec, q, b->q, S, b->st));
+    HIPCHK(launch_from_double(b->prec, ux, b->ux, S, b->st));
+    HIPCHK(launch_from_double(b->prec, uy, b->uy, S, b->st));
+    HIPCHK(launch_from_double(b->prec, uz, b->uz, S, b->st));
+    HIPCHK(launch_phi_split(b->prec, b->lp, phi, b->phi, b->phibar, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    b->direct = true;
+    b->ebuf = true;
+    b->initialised = true;
+    b->step = 0;
+    b->t = 0.0;
+    return EHD_OK;
+}
+
+int ehd_step(ehd_box* b, int nsteps)
+{
+    if (!b) return fail(EHD_EINVAL, "null box");
+    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    if (nsteps < 0) return fail(EHD_EINVAL, "nsteps < 0");
+    hipEvent_t s0 = nullptr, s1 = nullptr;
+    if (b->prof && nsteps > 0) {
+        s0 = take_event(b);
+        s1 = take_event(b);
+        if (!s0 || !s1) return fail(EHD_EHIP, "hipEventCreate failed");
+        HIPCHK(hipEventRecord(s0, b->st));
+    }
+    for (int i = 0; i < nsteps; ++i) {
+        int rc = run_lbm(b, i == nsteps - 1);
+        if (rc) return rc;
+        rc = run_poisson(b);
+        if (rc) return rc;
+        b->step += 1;
+        b->t = b->t + b->der.dt;   // main.cu:208
+    }
+    if (b->prof && nsteps > 0) {
+        HIPCHK(hipEventRecord(s1, b->st));
+        b->step_pairs.emplace_back(s0, s1);
+        int rc = resolve_events(b);
+        if (rc) return rc;
+    }
+    return EHD_OK;
+}
+
+int ehd_lbm_step(ehd_box* b)
+{
+    if (!b) return fail(EHD_EINVAL, "null box");
+    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    return run_lbm(b, true);
+}
+
+int ehd_poisson(ehd_box* b)
+{
+    if (!b) return fail(EHD_EINVAL, "null box");
+    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    return run_poisson(b);
+}
+
+int ehd_read_fields(ehd_box* b, double* rho, double* q, double* phi, double* ux, double* uy, double* uz, double* ex,
+                    double* ey, double* ez)
+{
+    if (!b) return fail(EHD_EINVAL, "null box");
+    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    const size_t S = b->S;
+    double* tmp = nullptr;
+    int rc = alloc((void**)&tmp, S * sizeof(double));
+    if (rc) return rc;
+    struct Guard {
+        double* p;
+        ~Guard() { if (p) (void)hipFree(p); }
+    } guard{tmp};
+    struct Item {
+        double* host;
+        void* dev;
+    } items[] = {{rho, b->rho}, {q, b->q}, {ux, b->ux}, {uy, b->uy}, {uz, b->uz}};
+    for (auto& it : items) {
+        if (!it.host) continue;
+        HIPCHK(launch_to_double(b->prec, it.dev, tmp, S, b->st));
+        HIPCHK(hipMemcpyAsync(it.host, tmp, S * 8, hipMemcpyDeviceToHost, b->st));
+        HIPCHK(hipStreamSynchronize(b->st));
+    }
+    if (phi) {
+        HIPCHK(launch_phi_full(b->prec, b->lp, b->phi, b->phibar, tmp, b->st));
+        HIPCHK(hipMemcpyAsync(phi, tmp, S * 8, hipMemcpyDeviceToHost, b->st));
+        HIPCHK(hipStreamSynchronize(b->st));
+    }
+    if (ex || ey || ez) {
+        if (!b->ebuf) HIPCHK(launch_efield(b->prec, b->lp, b->phi, b->phibar, b->ex, b->ey, b->ez, b->st));
+        if (ex) HIPCHK(hipMemcpyAsync(ex, b->ex, S * 8, hipMemcpyDeviceToHost, b->st));
+        if (ey) HIPCHK(hipMemcpyAsync(ey, b->ey, S * 8, hipMemcpyDeviceToHost, b->st));
+        if (ez) HIPCHK(hipMemcpyAsync(ez, b->ez, S * 8, hipMemcpyDeviceToHost, b->st));
+        HIPCHK(hipStreamSynchronize(b->st));
+    }
+    return EHD_OK;
+}
+
+int ehd_write_phi(ehd_box* b, const double* phi)
+{
+    if (!b || !phi) return fail(EHD_EINVAL, "null argument");
+    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    double* tmp = nullptr;
+    int rc = alloc((void**)&tmp, b->S * sizeof(double));
+    if (rc) return rc;
+    struct Guard {
+        double* p;
+        ~Guard() { if (p) (void)hipFree(p); }
+    } guard{tmp};
+    HIPCHK(hipMemcpyAsync(tmp, phi, b->S * 8, hipMemcpyHostToDevice, b->st));
+    HIPCHK(launch_phi_split(b->prec, b->lp, tmp, b->phi, b->phibar, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    b->ebuf = false;
+    return EHD_OK;
+}
+
+int ehd_write_charge(ehd_box* b, const double* q)
+{
+    if (!b || !q) return fail(EHD_EINVAL, "null argument");
+    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    double* tmp = nullptr;
+    int rc = alloc((void**)&tmp, b->S * sizeof(double));
+    if (rc) return rc;
+    struct Guard {
+        double* p;
+        ~Guard() { if (p) (void)hipFree(p); }
+    } guard{tmp};
+    HIPCHK(hipMemcpyAsync(tmp, q, b->S * 8, hipMemcpyHostToDevice, b->st));
+    HIPCHK(launch_from_double(b->prec, tmp, b->q, b->S, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    return EHD_OK;
+}
+
+int ehd_read_populations(ehd_box* b, double* f0, double* f1, double* h0, double* h1)
+{
+    if (!b) return fail(EHD_EINVAL, "null box");
+    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    const size_t S = b->S;
+    double* tmp = nullptr;
+    int rc = alloc((void**)&tmp, 54 * S * sizeof(double));
+    if (rc) return rc;
+    struct Guard {
+        double* p;
+        ~Guard() { if (p) (void)hipFree(p); }
+    } guard{tmp};
+    double *d_f1 = tmp, *d_h1 = tmp + 26 * S, *d_f0 = tmp + 52 * S, *d_h0 = tmp + 53 * S;
+    const int A = b->cur;
+    HIPCHK(launch_read_pre(b->prec, b->lp, b->direct, b->f[A], b->f0[A], b->h[A], b->h0[A], d_f1, d_f0, d_h1, d_h0,
+                           b->st));
+    if (f0) HIPCHK(hipMemcpyAsync(f0, d_f0, S * 8, hipMemcpyDeviceToHost, b->st));
+    if (f1) HIPCHK(hipMemcpyAsync(f1, d_f1, 26 * S * 8, hipMemcpyDeviceToHost, b->st));
+    if (h0) HIPCHK(hipMemcpyAsync(h0, d_h0, S * 8, hipMemcpyDeviceToHost, b->st));
+    if (h1) HIPCHK(hipMemcpyAsync(h1, d_h1, 26 * S * 8, hipMemcpyDeviceToHost, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    return EHD_OK;
+}
+
+int ehd_write_populations(ehd_box* b, const double* f0, const double* f1, const double* h0, const double* h1)
+{
+    if (!b || !f0 || !f1 || !h0 || !h1) return fail(EHD_EINVAL, "null argument");
+    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    const size_t S = b->S;
+    double* tmp = nullptr;
+    int rc = alloc((void**)&tmp, 54 * S * sizeof(double));
+    if (rc) return rc;
+    struct Guard {
+        double* p;
+        ~Guard() { if (p) (void)hipFree(p); }
+    } guard{tmp};
+    double *d_f1 = tmp, *d_h1 = tmp + 26 * S, *d_f0 = tmp + 52 * S, *d_h0 = tmp + 53 * S;
+    HIPCHK(hipMemcpyAsync(d_f0, f0, S * 8, hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipMemcpyAsync(d_f1, f1, 26 * S * 8, hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipMemcpyAsync(d_h0, h0, S * 8, hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipMemcpyAsync(d_h1, h1, 26 * S * 8, hipMemcpyHostToDevice, b->st));
+    const int A = b->cur, B = 1 - b->cur;
+    HIPCHK(launch_write_pre(b->prec, b->lp, d_f1, d_f0, d_h1, d_h0, b->f[A], b->f0[A], b->h[A], b->h0[A], b->st));
+    // frozen wall rest populations live in both buffers
+    HIPCHK(hipMemcpyAsync(b->f0[B], b->f0[A], S * b->esz, hipMemcpyDeviceToDevice, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    b->direct = true;
+    return EHD_OK;
+}
+
+int ehd_set_nusselt_reference(ehd_box* b, double current_hs)
+{
+    if (!b) return fail(EHD_EINVAL, "null box");
+    b->current_hs = current_hs;
+    return EHD_OK;
+}
+
+int ehd_diagnostics(ehd_box* b, ehd_diag* out)
+{
+    if (!b || !out) return fail(EHD_EINVAL, "null argument");
+    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    HIPCHK(launch_diag(b->prec, b->lp, b->q, b->uz, b->rho, b->phi, b->phibar, b->diag_scr, b->diag_out, b->st));
+    double r[8] = {0};
+    HIPCHK(hipMemcpyAsync(r, b->diag_out, 5 * sizeof(double), hipMemcpyDeviceToHost, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    const ehd_derived& d = b->der;
+    out->current = r[0] * d.K * d.dy * d.dy;   // LBM.cu:1669
+    out->total_charge = r[1] * d.dx * d.dy * d.dz;
+    out->umax = r[2];
+    out->mean_rho = r[3] / ((double)b->nxy * (double)(b->nz - 2));
+    out->nusselt = (b->current_hs != 0.0) ? out->current / b->current_hs : 0.0;
+    out->step = b->step;
+    out->time = b->t;
+    if (r[4] != 0.0) return fail(EHD_ENONFINITE, "non-finite charge or velocity at %.0f nodes (step %lld)", r[4], b->step);
+    return EHD_OK;
+}
+
+int ehd_profile_enable(ehd_box* b, int on)
+{
+    if (!b) return fail(EHD_EINVAL, "null box");
+    b->prof = (on != 0);
+    return EHD_OK;
+}
+
+int ehd_profile_read(ehd_box* b, ehd_profile* out)
+{
+    if (!b || !out) return fail(EHD_EINVAL, "null argument");
+    int rc = resolve_events(b);
+    if (rc) return rc;
+    out->lbm_ms = b->lbm_ms;
+    out->lbm_launches = b->lbm_n;
+    out->poisson_ms = b->poi_ms;
+    out->poisson_calls = b->poi_n;
+    out->step_ms = b->step_ms;
+    out->steps = b->step_n;
+    return EHD_OK;
+}
+
+int ehd_profile_reset(ehd_box* b)
+{
+    if (!b) return fail(EHD_EINVAL, "null box");
+    int rc = resolve_events(b);
+    if (rc) return rc;
+    b->lbm_ms = b->poi_ms = b->step_ms = 0.0;
+    b->lbm_n = b->poi_n = b->step_n = 0;
+    return EHD_OK;
+}
+
+int ehd_sync(ehd_box* b)
+{
+    if (!b) return fail(EHD_EINVAL, "null box");
+    HIPCHK(hipStreamSynchronize(b->st));
+    return EHD_OK;
+}
+
+}  // extern "C"

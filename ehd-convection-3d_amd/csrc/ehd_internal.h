@@ -1,0 +1,102 @@
+// ehd_internal.h -- types shared by the HIP kernels and the C-ABI host layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/ehd.h"
+#include "ehd_lattice.h"
+
+namespace ehd {
+
+// Per-box constants handed to the kernels by value.
+struct LbmParams {
+    int nx, ny, nz;
+    unsigned nxy, S;             // nx*ny, nx*ny*nz
+    double dt, cfl, cs_square, K, exf, charge0, rho0;
+    double w[4];                 // w0, ws, wa, wd (LBM.h:73-76)
+    double omega_plus, omega_minus, omega_c_plus, omega_c_minus;  // LBM.cu:298-302
+    double multi[4];             // moving-wall terms per weight class (LBM.cu:1137-1139)
+    double inj[4];               // 2*charge0*w (LBM.cu:1288-1291)
+    double dx, dy, dz;
+    double q_ref;                // FP32S charge shift
+};
+
+// Device buffers of the Poisson solve.
+struct PoissonBufs {
+    int nx, ny, nz, nxh;         // nxh = nx/2+1
+    void* spec;                  // complex [nz-2][ny][nxh], element type of the path
+    double* dscr;                // FP64 complex forward-sweep values [nz-2][ny][nxh]
+    const double* cpt;           // Thomas pivots 1/(b - c'), [nz-2][ny][nxh]
+    const void* twx;             // nx roots exp(-2 pi i j/nx), element type of the path
+    const void* twy;             // ny roots
+    double* phibar;              // FP32S: xy-mean potential per plane [nz] (FP64)
+};
+
+enum class Prec { F64 = 0, F32S = 1 };
+
+// --- launch wrappers (lbm_f64.hip) ---------------------------------------
+struct PopBufs {
+    void* f;  void* f0;  void* h;  void* h0;     // state read (A)
+    void* fo; void* f0o; void* ho; void* h0o;    // state written (B)
+};
+struct MacroBufs {
+    void* rho; void* ux; void* uy; void* uz; void* q;   // element type of the path
+    const void* phi;                                     // potential (path type)
+    const double* phibar;                                // FP32S plane means (nullptr for F64)
+    const double* ex; const double* ey; const double* ez;   // explicit E (first step)
+};
+
+hipError_t launch_lbm_f64(const LbmParams& P, const PopBufs& pb, const MacroBufs& mb,
+                          bool first, bool ebuf, bool macro, hipStream_t st);
+hipError_t launch_lbm_f32s(const LbmParams& P, const PopBufs& pb, const MacroBufs& mb,
+                           bool first, bool ebuf, bool macro, hipStream_t st);
+
+// init_equilibrium (LBM.cu:98-276) from double macroscopic fields into the
+// pre-collision ("direct") representation of the path.
+hipError_t launch_init_equilibrium(Prec prec, const LbmParams& P, const double* rho, const double* q,
+                                   const double* ux, const double* uy, const double* uz,
+                                   const double* ex, const double* ey, const double* ez,
+                                   void* f, void* f0, void* h, void* h0, hipStream_t st);
+
+// Pre-collision populations of the reference (after stream + bc_charge) as
+// doubles, from either representation.
+hipError_t launch_read_pre(Prec prec, const LbmParams& P, bool direct, const void* f, const void* f0,
+                           const void* h, const void* h0, double* f1o, double* f0o, double* h1o,
+                           double* h0o, hipStream_t st);
+hipError_t launch_write_pre(Prec prec, const LbmParams& P, const double* f1, const double* f0,
+                            const double* h1, const double* h0, void* f, void* f0o, void* h, void* h0o,
+                            hipStream_t st);
+
+// E = -grad(phi) with the wall copy of Ez (poisson.cu:32-73), as doubles.
+hipError_t launch_efield(Prec prec, const LbmParams& P, const void* phi, const double* phibar,
+                         double* ex, double* ey, double* ez, hipStream_t st);
+
+// Macroscopic field conversion helpers.
+hipError_t launch_to_double(Prec prec, const void* src, double* dst, size_t n, hipStream_t st);
+hipError_t launch_from_double(Prec prec, const double* src, void* dst, size_t n, hipStream_t st);
+hipError_t launch_phi_full(Prec prec, const LbmParams& P, const void* phi, const double* phibar,
+                           double* out, hipStream_t st);
+hipError_t launch_phi_split(Prec prec, const LbmParams& P, const double* phi, void* phio,
+                            double* phibar, hipStream_t st);
+
+// Reference and hydrostatic initial fields (doubles).
+hipError_t launch_init_fields(const LbmParams& P, int mode, const double* qz, const double* phiz,
+                              uint64_t seed, double amp, double voltage, double lz, double* rho,
+                              double* q, double* phi, double* ux, double* uy, double* uz,
+                              hipStream_t st);
+int diag_blocks();
+
+// --- Poisson (poisson.hip) -------------------------------------------------
+hipError_t launch_poisson(Prec prec, const LbmParams& P, const PoissonBufs& pb, const void* q,
+                          void* phi, double voltage, double eps, hipStream_t st);
+
+// --- diagnostics (diag.hip) --------------------------------------------------
+// out[0]=sum_xy q_top*Ez_top, out[1]=sum interior q, out[2]=max uz (top extrapolated),
+// out[3]=sum interior rho, out[4]=nonfinite count
+hipError_t launch_diag(Prec prec, const LbmParams& P, const void* q, const void* uz, const void* rho,
+                       const void* phi, const double* phibar, double* scratch, double* out,
+                       hipStream_t st);
+
+}  // namespace ehd

@@ -1,0 +1,298 @@
+// lbm_f32s.hip -- fused pull-scheme LBM step with FP32 storage of shifted
+// populations (the performance path).
+//
+// Storage: g_d = f_d - w_d*rho0 and k_d = h_d - w_d*q_ref (q_ref = charge0),
+// float, SoA [d-1][z][y][x], double buffered; same wall and representation
+// rules as the FP64 kernel (lbm_f64.hip), which restates the reference step
+// (LBM.cu:278-1353) in its own operation order.  Here the collision is
+// written directly on the deviations, pair by pair (d, d+1 are opposites):
+//   rho = rho0 + sum g,  q = q_ref + sum k,
+//   j_a = sum_d c_da g_d,  u = (j/CFL + F dt/2)/rho,  F = q E (+ exf x),
+//   t_d = c_d.u /(CFL cs^2),
+//   geq+ = w (rho - rho0 + rho (t^2/2 - u^2/(2 cs^2))),  geq- = w rho t,
+//   F+  = w (c.u c.F /(CFL^2 cs^4) - u.F/cs^2),      F-  = w c.F /(CFL cs^2),
+//   g'_d = g_d - om+ (g+ - geq+) - om- (g- - geq-) + dt (s+ F+ + s- F-),
+// which is the algebra of gpu_collide_save (LBM.cu:519-1089) with the shift
+// cancelling in every difference (w_d = w_opp(d), sum_d w_d = 1).  h uses
+// u + K E, no source, and the charge rates.  Arithmetic is FP32; the large
+// parts (w*rho0, w*q_ref, the wall potential) never enter a float rounding.
+#include <hip/hip_runtime.h>
+
+#include "ehd_internal.h"
+
+namespace ehd {
+namespace {
+
+struct F32Consts {
+    float dt, cflinv, A;      // A = 1/(CFL cs^2)
+    float inv2cs2;            // 1/(2 cs^2)
+    float B;                  // 1/(CFL^2 cs^4)
+    float ics2;               // 1/cs^2
+    float K, exf, rho0, qref;
+    float omp, omm, ocp, ocm; // omega*dt
+    float sp, sm;             // 1 - omega dt/2
+    float w[4];
+    float multi[4];
+    float injk[4];            // 2 w (charge0 - q_ref)
+    float hdx, hdy, hdz;      // 0.5/dx ...
+};
+
+template <bool FIRST, bool EBUF>
+__global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c, PopBufs pb, MacroBufs mb,
+                                                       int macro, unsigned nblk_xcd)
+{
+    // XCD-aware block order: hardware block i runs on XCD i%8; give each XCD a
+    // contiguous range of rows so the phi stencil of neighbouring rows stays
+    // in that XCD's L2.
+    unsigned b = blockIdx.x;
+    if (nblk_xcd) b = (b & 7u) * nblk_xcd + (b >> 3);
+    const unsigned s = b * blockDim.x + threadIdx.x;
+    if (s >= P.S) return;
+    const unsigned nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy, S = P.S;
+    const unsigned x = s % nx, t = s / nx, y = t % ny, z = t / ny;
+    const unsigned xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
+    const unsigned ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
+    const unsigned zm = (z == 0) ? nz - 1 : z - 1, zp = (z == nz - 1) ? 0 : z + 1;
+    const unsigned XS[3] = {xm, x, xp};
+    const unsigned YS[3] = {ym * nx, y * nx, yp * nx};
+    const unsigned ZS[3] = {zm * nxy, z * nxy, zp * nxy};
+
+    const float* __restrict__ fA = (const float*)pb.f;
+    const float* __restrict__ f0A = (const float*)pb.f0;
+    const float* __restrict__ hA = (const float*)pb.h;
+    const float* __restrict__ h0A = (const float*)pb.h0;
+
+    float g[27], k[27];
+    g[0] = f0A[s];
+    if (FIRST) {
+        k[0] = h0A[s];
+#pragma unroll
+        for (int d = 1; d < 27; ++d) {
+            g[d] = fA[(d - 1) * S + s];
+            k[d] = hA[(d - 1) * S + s];
+        }
+    } else {
+#pragma unroll
+        for (int d = 1; d < 27; ++d) g[d] = fA[(d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]];
+        if (z == 0) {
+            k[0] = -h0A[s] + c.injk[0];
+#pragma unroll
+            for (int d = 1; d < 27; ++d) k[d] = -hA[(opp(d) - 1) * S + s] + c.injk[WCLS[d]];
+        } else {
+            k[0] = h0A[s];
+#pragma unroll
+            for (int d = 1; d < 27; ++d) k[d] = hA[(d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]];
+        }
+    }
+
+    // E = -grad(phi): float fluctuation plus FP64 plane means
+    const float* __restrict__ phi = (const float*)mb.phi;
+    float Ex, Ey, Ez;
+    const unsigned yc = y * nx, zc = z * nxy;
+    if (EBUF) {
+        Ex = (float)mb.ex[s]; Ey = (float)mb.ey[s]; Ez = (float)mb.ez[s];
+    } else {
+        Ex = c.hdx * (phi[xm + yc + zc] - phi[xp + yc + zc]);
+        Ey = c.hdy * (phi[x + ym * nx + zc] - phi[x + yp * nx + zc]);
+        const unsigned ze = (z == 0) ? 1u : ((z == nz - 1) ? nz - 2 : z);
+        const float dmn = (float)(mb.phibar[ze - 1] - mb.phibar[ze + 1]);
+        Ez = c.hdz * ((phi[x + yc + (ze - 1) * nxy] - phi[x + yc + (ze + 1) * nxy]) + dmn);
+    }
+
+    // moments
+    float drho = g[0], dq = k[0];
+#pragma unroll
+    for (int d = 1; d < 27; d += 2) {
+        drho += g[d] + g[d + 1];
+        dq += k[d] + k[d + 1];
+    }
+    const float rho = c.rho0 + drho;
+    const float charge = c.qref + dq;
+    float j[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float pp = 0.f, nn = 0.f;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            pp += g[MOM[a][i]];
+            nn += g[MOM[a][9 + i]];
+        }
+        j[a] = pp - nn;
+    }
+    const float Fx = charge * Ex + c.exf, Fy = charge * Ey, Fz = charge * Ez;
+    const float hdt = 0.5f * c.dt;
+    const float rinv = 1.0f / rho;
+    float ux = rinv * (j[0] * c.cflinv + Fx * hdt);
+    float uy = rinv * (j[1] * c.cflinv + Fy * hdt);
+    float uz = rinv * (j[2] * c.cflinv + Fz * hdt);
+
+    if (z == 0) {
+        // velocity override from the pre-collision populations of z=1 (LBM.cu:416-493)
+        float gm[27];
+        float dqm;
+        const unsigned s1 = s + nxy;
+        dqm = h0A[s1];
+        if (FIRST) {
+#pragma unroll
+            for (int d = 1; d < 27; ++d) {
+                gm[d] = fA[(d - 1) * S + s1];
+                dqm += hA[(d - 1) * S + s1];
+            }
+        } else {
+            const unsigned Z1[3] = {0u, nxy, 2u * nxy};
+#pragma unroll
+            for (int d = 1; d < 27; ++d) {
+                const unsigned src = XS[1 - CX[d]] + YS[1 - CY[d]] + Z1[1 - CZ[d]];
+                gm[d] = fA[(d - 1) * S + src];
+                dqm += hA[(d - 1) * S + src];
+            }
+        }
+        float Exm, Eym, Ezm;
+        if (EBUF) {
+            Exm = (float)mb.ex[s1]; Eym = (float)mb.ey[s1]; Ezm = (float)mb.ez[s1];
+        } else {
+            Exm = c.hdx * (phi[xm + yc + nxy] - phi[xp + yc + nxy]);
+            Eym = c.hdy * (phi[x + ym * nx + nxy] - phi[x + yp * nx + nxy]);
+            const float dmn = (float)(mb.phibar[0] - mb.phibar[2]);
+            Ezm = c.hdz * ((phi[x + yc] - phi[x + yc + 2 * nxy]) + dmn);
+        }
+        const float qm = c.qref + dqm;
+        float jm[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float pp = 0.f, nn = 0.f;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                pp += gm[MOM[a][i]];
+                nn += gm[MOM[a][9 + i]];
+            }
+            jm[a] = pp - nn;
+        }
+        ux = -rinv * (jm[0] * c.cflinv + (qm * Exm + c.exf) * hdt);
+        uy = -rinv * (jm[1] * c.cflinv + (qm * Eym) * hdt);
+        uz = -rinv * (jm[2] * c.cflinv + (qm * Ezm) * hdt);
+    }
+
+    if (macro) {
+        ((float*)mb.rho)[s] = rho;
+        ((float*)mb.ux)[s] = ux;
+        ((float*)mb.uy)[s] = uy;
+        ((float*)mb.uz)[s] = uz;
+    }
+    ((float*)mb.q)[s] = charge;
+
+    const float ucx = ux + c.K * Ex, ucy = uy + c.K * Ey, ucz = uz + c.K * Ez;
+    const float U2 = (ux * ux + uy * uy + uz * uz) * c.inv2cs2;
+    const float Uc2 = (ucx * ucx + ucy * ucy + ucz * ucz) * c.inv2cs2;
+    const float uF = (ux * Fx + uy * Fy + uz * Fz) * c.ics2;
+
+    float* __restrict__ fB = (float*)pb.fo;
+    float* __restrict__ f0B = (float*)pb.f0o;
+    float* __restrict__ hB = (float*)pb.ho;
+    float* __restrict__ h0B = (float*)pb.h0o;
+    const bool fwall = (z == 0) || (z == nz - 1);
+
+    // rest populations
+    {
+        const float geq0 = c.w[0] * (drho - rho * U2);
+        const float keq0 = c.w[0] * (dq - charge * Uc2);
+        const float F0 = -c.w[0] * uF;
+        const float g0n = g[0] - c.omp * (g[0] - geq0) + c.dt * c.sp * F0;
+        const float k0n = k[0] - c.ocp * (k[0] - keq0);
+        f0B[s] = fwall ? g[0] : g0n;
+        if (z != nz - 1) {
+            h0B[s] = k0n;
+            if (z == nz - 2) h0B[s + nxy] = k0n;
+        }
+    }
+#pragma unroll
+    for (int d = 1; d < 27; d += 2) {
+        const int e = d + 1;
+        const float w = c.w[WCLS[d]];
+        const float cu = (float)CX[d] * ux + (float)CY[d] * uy + (float)CZ[d] * uz;
+        const float cc = (float)CX[d] * ucx + (float)CY[d] * ucy + (float)CZ[d] * ucz;
+        const float cF = (float)CX[d] * Fx + (float)CY[d] * Fy + (float)CZ[d] * Fz;
+        const float tt = c.A * cu, tc = c.A * cc;
+        const float geqp = w * (drho + rho * (0.5f * tt * tt - U2));
+        const float geqm = w * rho * tt;
+        const float keqp = w * (dq + charge * (0.5f * tc * tc - Uc2));
+        const float keqm = w * charge * tc;
+        const float Fp = w * (c.B * cu * cF - uF);
+        const float Fm = w * (c.A * cF);
+        const float gp = 0.5f * (g[d] + g[e]), gmn = 0.5f * (g[d] - g[e]);
+        const float kp = 0.5f * (k[d] + k[e]), kmn = 0.5f * (k[d] - k[e]);
+        const float rp = c.omp * (gp - geqp), rm = c.omm * (gmn - geqm);
+        const float sP = c.dt * c.sp * Fp, sM = c.dt * c.sm * Fm;
+        const float qp = c.ocp * (kp - keqp), qm2 = c.ocm * (kmn - keqm);
+        const float gd = g[d] - rp - rm + sP + sM;
+        const float ge = g[e] - rp + rm + sP - sM;
+        const float kd = k[d] - qp - qm2;
+        const float ke = k[e] - qp + qm2;
+        // f: bounce-back of the pre-collision values at the walls
+        float od, oe;
+        if (z == 0) {
+            od = g[e];
+            oe = g[d];
+        } else if (z == nz - 1) {
+            od = g[e] + (float)TOPSGN[d] * c.multi[WCLS[d]];
+            oe = g[d] + (float)TOPSGN[e] * c.multi[WCLS[d]];
+        } else {
+            od = gd;
+            oe = ge;
+        }
+        fB[(d - 1) * S + s] = od;
+        fB[(e - 1) * S + s] = oe;
+        if (z != nz - 1) {
+            hB[(d - 1) * S + s] = kd;
+            hB[(e - 1) * S + s] = ke;
+            if (z == nz - 2) {
+                hB[(d - 1) * S + s + nxy] = kd;
+                hB[(e - 1) * S + s + nxy] = ke;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_lbm_f32s(const LbmParams& P, const PopBufs& pb, const MacroBufs& mb, bool first, bool ebuf,
+                           bool macro, hipStream_t st)
+{
+    F32Consts c;
+    const double cs2 = P.cs_square, cfl = P.cfl;
+    c.dt = (float)P.dt;
+    c.cflinv = (float)(1.0 / cfl);
+    c.A = (float)(1.0 / (cfl * cs2));
+    c.inv2cs2 = (float)(0.5 / cs2);
+    c.B = (float)(1.0 / (cfl * cfl * cs2 * cs2));
+    c.ics2 = (float)(1.0 / cs2);
+    c.K = (float)P.K;
+    c.exf = (float)P.exf;
+    c.rho0 = (float)P.rho0;
+    c.qref = (float)P.q_ref;
+    c.omp = (float)(P.omega_plus * P.dt);
+    c.omm = (float)(P.omega_minus * P.dt);
+    c.ocp = (float)(P.omega_c_plus * P.dt);
+    c.ocm = (float)(P.omega_c_minus * P.dt);
+    c.sp = (float)(1.0 - 0.5 * P.dt * P.omega_plus);
+    c.sm = (float)(1.0 - 0.5 * P.dt * P.omega_minus);
+    for (int i = 0; i < 4; ++i) {
+        c.w[i] = (float)P.w[i];
+        c.multi[i] = (float)P.multi[i];
+        c.injk[i] = (float)(2.0 * P.w[i] * (P.charge0 - P.q_ref));
+    }
+    c.hdx = (float)(0.5 / P.dx);
+    c.hdy = (float)(0.5 / P.dy);
+    c.hdz = (float)(0.5 / P.dz);
+    const unsigned nblk = (P.S + 255) / 256;
+    const unsigned nblk_xcd = (nblk % 8 == 0) ? nblk / 8 : 0;
+    dim3 g(nblk), b(256);
+    int m = macro ? 1 : 0;
+    if (first && ebuf) lbm_f32s_kernel<true, true><<<g, b, 0, st>>>(P, c, pb, mb, m, nblk_xcd);
+    else if (first) lbm_f32s_kernel<true, false><<<g, b, 0, st>>>(P, c, pb, mb, m, nblk_xcd);
+    else if (ebuf) lbm_f32s_kernel<false, true><<<g, b, 0, st>>>(P, c, pb, mb, m, nblk_xcd);
+    else lbm_f32s_kernel<false, false><<<g, b, 0, st>>>(P, c, pb, mb, m, nblk_xcd);
+    return hipGetLastError();
+}
+
+}  // namespace ehd

@@ -1,0 +1,248 @@
+"""Python binding of the MI355X coupled-EHD-step library (``include/ehd.h``).
+
+A thin ctypes layer over ``lib/libehd_amd.so``: every call goes through the
+C-ABI, nothing here computes.  There is no fallback: if the shared library is
+missing the import of :func:`lib` raises, so a test or benchmark can never
+silently run anything but the HIP kernels.
+
+Field arrays use the reference layout (LBM.cu:21-34): scalars ``[z][y][x]``
+(numpy shape ``(nz, ny, nx)``), moving populations ``[d-1][z][y][x]``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libehd_amd.so")
+
+FP64, FP32S = 0, 1
+INIT_REFERENCE, INIT_HYDROSTATIC, INIT_FIELDS = 0, 1, 2
+ERRORS = {0: "EHD_OK", -1: "EHD_EINVAL", -2: "EHD_EHIP", -3: "EHD_ENOMEM", -4: "EHD_ESTATE",
+          -5: "EHD_ENONFINITE"}
+
+
+class EhdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("T", C.c_double), ("C", C.c_double), ("M", C.c_double), ("Fe", C.c_double),
+        ("Pr", C.c_double), ("lambda_f", C.c_double), ("lambda_c", C.c_double),
+        ("cfl", C.c_double), ("voltage", C.c_double), ("eps", C.c_double), ("rho0", C.c_double),
+        ("lz", C.c_double), ("uw", C.c_double), ("exf", C.c_double),
+        ("precision", C.c_int), ("init_mode", C.c_int),
+        ("seed", C.c_uint64), ("perturb_amp", C.c_double),
+    ]
+
+
+class Derived(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "dx", "dy", "dz", "dt", "lx", "ly", "cs_square", "K", "nu", "diffu", "charge0",
+        "tau_plus", "tau_minus", "tau_c_plus", "tau_c_minus")]
+
+
+class Diag(C.Structure):
+    _fields_ = [("current", C.c_double), ("total_charge", C.c_double), ("umax", C.c_double),
+                ("nusselt", C.c_double), ("mean_rho", C.c_double), ("step", C.c_longlong),
+                ("time", C.c_double)]
+
+
+class Profile(C.Structure):
+    _fields_ = [("lbm_ms", C.c_double), ("lbm_launches", C.c_longlong),
+                ("poisson_ms", C.c_double), ("poisson_calls", C.c_longlong),
+                ("step_ms", C.c_double), ("steps", C.c_longlong)]
+
+
+EXPORTS = (
+    "ehd_params_default", "ehd_derive", "ehd_compute_parameters", "ehd_create", "ehd_destroy",
+    "ehd_init", "ehd_step", "ehd_lbm_step", "ehd_poisson", "ehd_read_fields", "ehd_write_phi",
+    "ehd_write_charge", "ehd_read_populations", "ehd_write_populations", "ehd_diagnostics",
+    "ehd_set_nusselt_reference", "ehd_profile_enable", "ehd_profile_read", "ehd_profile_reset",
+    "ehd_sync", "ehd_last_error", "ehd_abi_version",
+)
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    L.ehd_params_default.argtypes = [C.POINTER(Params)]
+    L.ehd_derive.argtypes = [C.POINTER(Params), C.POINTER(Derived)]
+    L.ehd_compute_parameters.argtypes = [C.POINTER(Derived), C.POINTER(Params), dp, dp, dp, dp]
+    L.ehd_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    L.ehd_destroy.argtypes = [vp]
+    L.ehd_destroy.restype = None
+    L.ehd_init.argtypes = [vp, dp]
+    L.ehd_step.argtypes = [vp, C.c_int]
+    L.ehd_lbm_step.argtypes = [vp]
+    L.ehd_poisson.argtypes = [vp]
+    L.ehd_read_fields.argtypes = [vp] + [dp] * 9
+    L.ehd_write_phi.argtypes = [vp, dp]
+    L.ehd_write_charge.argtypes = [vp, dp]
+    L.ehd_read_populations.argtypes = [vp, dp, dp, dp, dp]
+    L.ehd_write_populations.argtypes = [vp, dp, dp, dp, dp]
+    L.ehd_diagnostics.argtypes = [vp, C.POINTER(Diag)]
+    L.ehd_set_nusselt_reference.argtypes = [vp, C.c_double]
+    L.ehd_profile_enable.argtypes = [vp, C.c_int]
+    L.ehd_profile_read.argtypes = [vp, C.POINTER(Profile)]
+    L.ehd_profile_reset.argtypes = [vp]
+    L.ehd_sync.argtypes = [vp]
+    L.ehd_last_error.argtypes = []
+    L.ehd_last_error.restype = C.c_char_p
+    L.ehd_abi_version.argtypes = []
+    for name in EXPORTS:
+        f = getattr(L, name)
+        if name not in ("ehd_destroy", "ehd_last_error"):
+            f.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().ehd_last_error().decode(errors="replace")
+        raise EhdError(rc, msg)
+
+
+def _dp(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def default_params() -> Params:
+    p = Params()
+    _check(lib().ehd_params_default(C.byref(p)))
+    return p
+
+
+def make_params(nx, ny, nz, T=200.0, C_=10.0, M=10.0, Fe=4000.0, Pr=10.0, lam=0.25, lam_c=1e-6,
+                precision=FP64, init_mode=INIT_HYDROSTATIC, seed=1, amp=1e-3,
+                cfl=0.01, voltage=1.0e4, eps=1.0e-4, rho0=1600.0, lz=1.0, uw=0.0, exf=0.0) -> Params:
+    p = default_params()
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.T, p.C, p.M, p.Fe, p.Pr = T, C_, M, Fe, Pr
+    p.lambda_f, p.lambda_c = lam, lam_c
+    p.cfl, p.voltage, p.eps, p.rho0, p.lz, p.uw, p.exf = cfl, voltage, eps, rho0, lz, uw, exf
+    p.precision, p.init_mode, p.seed, p.perturb_amp = precision, init_mode, seed, amp
+    return p
+
+
+def derive(p: Params) -> Derived:
+    d = Derived()
+    _check(lib().ehd_derive(C.byref(p), C.byref(d)))
+    return d
+
+
+def compute_parameters(d: Derived, p: Params):
+    T, M, Cc, Fe = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+    _check(lib().ehd_compute_parameters(C.byref(d), C.byref(p), C.byref(T), C.byref(M), C.byref(Cc), C.byref(Fe)))
+    return T.value, M.value, Cc.value, Fe.value
+
+
+class Box:
+    """One simulation box on the GPU (owns every device buffer)."""
+
+    def __init__(self, params: Params):
+        self.params = params
+        self.shape = (params.nz, params.ny, params.nx)
+        self.S = params.nx * params.ny * params.nz
+        h = C.c_void_p()
+        _check(lib().ehd_create(C.byref(params), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ehd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init(self, fields9: np.ndarray | None = None):
+        a = None
+        if fields9 is not None:
+            a = np.ascontiguousarray(fields9, dtype=np.float64)
+            if a.size != 9 * self.S:
+                raise ValueError("fields9 must hold 9*S doubles")
+        _check(lib().ehd_init(self._h, _dp(a)))
+
+    def step(self, n: int = 1):
+        _check(lib().ehd_step(self._h, int(n)))
+
+    def lbm_step(self):
+        _check(lib().ehd_lbm_step(self._h))
+
+    def poisson(self):
+        _check(lib().ehd_poisson(self._h))
+
+    def read_fields(self, names=("rho", "q", "phi", "ux", "uy", "uz", "ex", "ey", "ez")) -> dict:
+        order = ("rho", "q", "phi", "ux", "uy", "uz", "ex", "ey", "ez")
+        out = {n: np.empty(self.shape) for n in names}
+        _check(lib().ehd_read_fields(self._h, *[_dp(out.get(n)) for n in order]))
+        return out
+
+    def write_phi(self, phi: np.ndarray):
+        a = np.ascontiguousarray(phi, dtype=np.float64)
+        _check(lib().ehd_write_phi(self._h, _dp(a)))
+
+    def write_charge(self, q: np.ndarray):
+        a = np.ascontiguousarray(q, dtype=np.float64)
+        _check(lib().ehd_write_charge(self._h, _dp(a)))
+
+    def read_populations(self):
+        nz, ny, nx = self.shape
+        f0 = np.empty((nz, ny, nx)); h0 = np.empty((nz, ny, nx))
+        f1 = np.empty((26, nz, ny, nx)); h1 = np.empty((26, nz, ny, nx))
+        _check(lib().ehd_read_populations(self._h, _dp(f0), _dp(f1), _dp(h0), _dp(h1)))
+        return f0, f1, h0, h1
+
+    def write_populations(self, f0, f1, h0, h1):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (f0, f1, h0, h1)]
+        _check(lib().ehd_write_populations(self._h, *[_dp(a) for a in arrs]))
+
+    def diagnostics(self) -> Diag:
+        d = Diag()
+        _check(lib().ehd_diagnostics(self._h, C.byref(d)))
+        return d
+
+    def set_nusselt_reference(self, current_hs: float):
+        _check(lib().ehd_set_nusselt_reference(self._h, float(current_hs)))
+
+    def profile_enable(self, on: bool = True):
+        _check(lib().ehd_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self) -> Profile:
+        pr = Profile()
+        _check(lib().ehd_profile_read(self._h, C.byref(pr)))
+        return pr
+
+    def profile_reset(self):
+        _check(lib().ehd_profile_reset(self._h))
+
+    def sync(self):
+        _check(lib().ehd_sync(self._h))

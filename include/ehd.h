@@ -1,0 +1,161 @@
+/*
+ * ehd.h -- C-ABI of the MI355X implementation of the coupled EHD time step
+ * (D3Q27 TRT lattice Boltzmann for flow f and charge h, FFT Poisson solve for
+ * the potential, E = -grad(phi)).
+ *
+ * The reference has no FFI layer: its drop-in boundary is the set of host
+ * wrappers main() calls.  Each entry point below names the reference
+ * interface it replaces.  All functions return 0 on success and a negative
+ * EHD_E* code on failure; ehd_last_error() then holds a message (thread
+ * local).  Calls on one box must be serialised by the caller; distinct boxes
+ * may be driven from distinct threads.  Host buffers are caller owned, in the
+ * reference layout: scalar fields [z][y][x] (x fastest), moving populations
+ * [d-1][z][y][x] (LBM.cu:21-34), all double precision.
+ */
+#ifndef EHD_H
+#define EHD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EHD_ABI_VERSION 1
+
+/* error codes */
+#define EHD_OK             0
+#define EHD_EINVAL        -1   /* bad argument or parameter */
+#define EHD_EHIP          -2   /* HIP runtime error */
+#define EHD_ENOMEM        -3   /* device allocation failed */
+#define EHD_ESTATE        -4   /* call not valid in the box's state */
+#define EHD_ENONFINITE    -5   /* NaN/Inf detected by the diagnostics watchdog */
+
+/* storage / arithmetic of the populations */
+#define EHD_FP64           0   /* double storage, double arithmetic, reference operation order */
+#define EHD_FP32S          1   /* float storage of f - w*rho0 and h - w*q_ref, float arithmetic */
+
+/* initial state */
+#define EHD_INIT_REFERENCE   0 /* rho0, q=0, linear phi, u=0, E=0 (gpu_initialization, LBM.cu:80-96) */
+#define EHD_INIT_HYDROSTATIC 1 /* analytic unipolar-injection profile q(z), phi(z), E=-grad(phi),
+                                  plus seeded uniform perturbation of u in the interior */
+#define EHD_INIT_FIELDS      2 /* caller-supplied rho,q,phi,u,v,w,Ex,Ey,Ez (read_data, LBM.cu:1596-1643) */
+
+/* Driver surface of main.cu (grid, T, C, M, Fe, Pr, TRT magic parameter) plus
+ * the physical constants LBM.h fixes at compile time (LBM.h:31-80). */
+typedef struct ehd_params {
+    int    nx, ny, nz;        /* grid nodes; walls at z=0 and z=nz-1; periodic x,y (LBM.h:34-36) */
+    double T, C, M, Fe;       /* nondimensional groups (compute_parameters, LBM.cu:1356-1379) */
+    double Pr;                /* carried as metadata; T, C, M, Fe fix nu and D */
+    double lambda_f;          /* TRT magic parameter, flow   (LBM.h:79, 1/12 in the reference) */
+    double lambda_c;          /* TRT magic parameter, charge (LBM.h:80, 1e-6) */
+    double cfl;               /* dt/dx (LBM.h:51, 0.01) */
+    double voltage, eps, rho0, lz;   /* LBM.h:42,54-57 (1e4, 1e-4, 1600, 1) */
+    double uw, exf;           /* moving-top-wall speed, x body force (LBM.h:47-50; 0 as shipped) */
+    int    precision;         /* EHD_FP64 | EHD_FP32S */
+    int    init_mode;         /* EHD_INIT_* */
+    uint64_t seed;            /* perturbation seed (EHD_INIT_HYDROSTATIC) */
+    double perturb_amp;       /* perturbation amplitude of u, physical units (1e-3) */
+} ehd_params;
+
+/* Dimensional constants derived from ehd_params (inverse of compute_parameters). */
+typedef struct ehd_derived {
+    double dx, dy, dz, dt, lx, ly, cs_square;
+    double K, nu, diffu, charge0;
+    double tau_plus, tau_minus, tau_c_plus, tau_c_minus;
+} ehd_derived;
+
+/* Global FP64 diagnostics, reduced on the device. */
+typedef struct ehd_diag {
+    double current;       /* collector current, current() (LBM.cu:1646-1671) */
+    double total_charge;  /* sum of q over interior nodes times dx*dy*dz */
+    double umax;          /* max of uz with the top-wall extrapolation (record_umax, LBM.cu:1673-1713) */
+    double nusselt;       /* current / hydrostatic current of the same box (0 if not set) */
+    double mean_rho;      /* mean density over interior nodes */
+    long long step;       /* steps taken since init */
+    double time;          /* physical time t (main.cu:208) */
+} ehd_diag;
+
+typedef struct ehd_box ehd_box;
+
+/* Reference defaults (LBM.h): 122x122x101, T=996, M=10, C=10, Fe=4000,
+ * lambda 1/12 and 1e-6, FP64, reference initial state. */
+int ehd_params_default(ehd_params* p);
+
+/* Inverse of compute_parameters (LBM.cu:1356-1379) plus the TRT rates of
+ * gpu_collide_save (LBM.cu:298-302).  Host only, needs no GPU. */
+int ehd_derive(const ehd_params* p, ehd_derived* d);
+
+/* Forward map, compute_parameters (LBM.cu:1356-1379). */
+int ehd_compute_parameters(const ehd_derived* d, const ehd_params* p,
+                           double* T, double* M, double* C, double* Fe);
+
+/* Allocates every device buffer of one box (main.cu:79-107 allocations; the
+ * cuFFT plan of main.cu:104 becomes the library's own FFT plans). */
+int ehd_create(const ehd_params* p, ehd_box** out);
+void ehd_destroy(ehd_box* b);   /* main.cu:283-306 frees */
+
+/* Initial macroscopic state plus equilibrium populations:
+ * initialization/read_data then init_equilibrium (main.cu:153-164,
+ * LBM.cu:69-276).  fields9 (9*S doubles: rho,q,phi,u,v,w,Ex,Ey,Ez) is
+ * required for EHD_INIT_FIELDS and ignored otherwise. */
+int ehd_init(ehd_box* b, const double* fields9);
+
+/* nsteps coupled steps, each = stream_collide_save + extension + FFT +
+ * derivative + IFFT + extract + efield (main.cu:181-208). */
+int ehd_step(ehd_box* b, int nsteps);
+
+/* The LBM part alone, stream_collide_save (LBM.cu:278-293), with the E field
+ * of the current potential. */
+int ehd_lbm_step(ehd_box* b);
+
+/* The Poisson part alone on the box's current charge: extension, FFT,
+ * derivative, IFFT, extract (poisson.cu:78-167, main.cu:191-203). */
+int ehd_poisson(ehd_box* b);
+
+/* Copy macroscopic fields to the host; any pointer may be NULL.  Values are
+ * those of the last step (the wall rows are the raw collide outputs, as the
+ * reference stores them). E is -grad(phi) of the current potential
+ * (efield, poisson.cu:32-73). */
+int ehd_read_fields(ehd_box* b, double* rho, double* q, double* phi,
+                    double* ux, double* uy, double* uz,
+                    double* ex, double* ey, double* ez);
+
+/* Overwrite the potential (S doubles) or the charge (S doubles). */
+int ehd_write_phi(ehd_box* b, const double* phi);
+int ehd_write_charge(ehd_box* b, const double* q);
+
+/* Pre-collision populations between steps, i.e. the reference's f0, f1, h0,
+ * h1 after stream and bc_charge (LBM.cu:278-293).  f0,h0: S doubles; f1,h1:
+ * 26*S doubles.  Any pointer may be NULL. */
+int ehd_read_populations(ehd_box* b, double* f0, double* f1, double* h0, double* h1);
+int ehd_write_populations(ehd_box* b, const double* f0, const double* f1,
+                          const double* h0, const double* h1);
+
+/* FP64 reductions on the device (current, total charge, umax, ...). */
+int ehd_diagnostics(ehd_box* b, ehd_diag* out);
+
+/* Reference current of the hydrostatic state, used for the Nusselt number. */
+int ehd_set_nusselt_reference(ehd_box* b, double current_hs);
+
+/* Per-kernel timing with HIP events on the box's stream.  When enabled,
+ * every launch of the fused LBM kernel and of the Poisson chain is bracketed
+ * by events; ehd_profile_read returns accumulated milliseconds and counts. */
+typedef struct ehd_profile {
+    double lbm_ms;      long long lbm_launches;
+    double poisson_ms;  long long poisson_calls;
+    double step_ms;     long long steps;   /* wall time of ehd_step calls, event based */
+} ehd_profile;
+int ehd_profile_enable(ehd_box* b, int on);
+int ehd_profile_read(ehd_box* b, ehd_profile* out);
+int ehd_profile_reset(ehd_box* b);
+
+int ehd_sync(ehd_box* b);
+const char* ehd_last_error(void);
+int ehd_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,75 @@
+"""The C-ABI library loads and exports every entry point include/ehd.h declares;
+host-only entry points (parameter derivation) work without a GPU."""
+import os
+import re
+import subprocess
+
+import pytest
+
+import ehd_amd as E
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ehd.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ehd_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_declares_the_binding_exports():
+    assert declared_functions() == sorted(E.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", E.LIB_PATH], check=True, capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (ehd_[a-z_0-9]+)$", out, flags=re.M))
+    missing = [f for f in declared_functions() if f not in exported]
+    assert not missing, missing
+    L = E.lib()
+    for f in declared_functions():
+        assert hasattr(L, f)
+
+
+def test_abi_version():
+    assert E.lib().ehd_abi_version() == 1
+
+
+def test_params_default_match_reference_header():
+    p = E.default_params()
+    assert (p.nx, p.ny, p.nz) == (122, 122, 101)          # LBM.h:34-36
+    assert abs(p.T - 996.0000637) < 1e-6 and abs(p.M - 10) < 1e-12 and abs(p.C - 10) < 1e-12
+    assert abs(p.Fe - 4000) < 1e-9
+    assert p.lambda_f == 1.0 / 12.0 and p.lambda_c == 1e-6   # LBM.h:79-80
+    d = E.derive(p)
+    assert abs(d.K - 2.5e-5) < 1e-18 and abs(d.nu / 0.0251004 - 1) < 1e-14
+    assert abs(d.diffu - 6.25e-5) < 1e-18 and abs(d.charge0 - 10.0) < 1e-13
+    assert d.dz == 0.01 and abs(d.dt - 1e-4) < 1e-19 and abs(d.lx - 1.22) < 1e-15
+
+
+def test_derive_matches_oracle():
+    from oracle import pyoracle as O
+    for (nx, ny, nz, T, C) in ((64, 64, 32, 200.0, 10.0), (256, 256, 128, 400.0, 10.0), (48, 48, 32, 150.0, 5.0)):
+        p = E.make_params(nx, ny, nz, T=T, C_=C)
+        d = E.derive(p)
+        o = O.make_params(nx, ny, nz, T=T, M=10.0, C_=C, Fe=4000.0, lam=0.25)
+        for a, b in (("dx", "dx"), ("dz", "dz"), ("dt", "dt"), ("lx", "lx"), ("cs_square", "cs_square"),
+                     ("K", "K"), ("nu", "nu"), ("diffu", "diffu"), ("charge0", "charge0")):
+            assert getattr(d, a) == getattr(o, b), a
+        T2, M2, C2, Fe2 = E.compute_parameters(d, p)
+        assert abs(T2 / T - 1) < 1e-14 and abs(C2 / C - 1) < 1e-14 and abs(M2 / 10 - 1) < 1e-14
+        assert abs((d.tau_plus - 0.5) * (d.tau_minus - 0.5) - 0.25) < 1e-12
+
+
+@pytest.mark.parametrize("bad", [dict(nx=7), dict(ny=0), dict(nz=3), dict(T=-1.0), dict(precision=5),
+                                 dict(init_mode=9), dict(lam=0.0)])
+def test_invalid_params_rejected_on_host(bad):
+    kw = dict(nx=8, ny=8, nz=8)
+    nx, ny, nz = bad.pop("nx", 8), bad.pop("ny", 8), bad.pop("nz", 8)
+    p = E.make_params(nx, ny, nz, **bad)
+    with pytest.raises(E.EhdError) as ei:
+        E.derive(p)
+    assert ei.value.code == -1
+    assert E.lib().ehd_last_error()
+    del kw

@@ -1,0 +1,239 @@
+"""GPU parity of the HIP coupled step against the FP64 CPU oracle, through the C-ABI.
+
+Bars:
+  * FP64 LBM part (fused pull collide/boundary/stream/injection): bit-identical to
+    the oracle's restatement of LBM.cu:278-1353 on the same state, any number of steps;
+  * FP64 initial equilibria and E field: bit-identical;
+  * Poisson: FP64 relative max error <= 1e-13 against the oracle's odd-extension FFT
+    solve (a different algorithm: x,y FFT plus tridiagonal z solve on the GPU);
+    FP32S <= 2e-6;
+  * coupled FP64 steps: rel-L2 on u, q (interior) and phi against the oracle.
+"""
+import numpy as np
+import pytest
+
+import ehd_amd as E
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = (O.RHO, O.Q, O.PHI, O.UX, O.UY, O.UZ, O.EX, O.EY, O.EZ)
+
+
+def oracle_box(nx, ny, nz, T=200.0, C=10.0, lam=0.25, uw=0.0, exf=0.0, seed=1, amp=1e-3, mode="hydro"):
+    ob = O.Box(O.make_params(nx, ny, nz, T=T, M=10.0, C_=C, Fe=4000.0, lam=lam, uw=uw, exf=exf))
+    ob.groups = dict(T=T, C_=C, M=10.0, Fe=4000.0)
+    if mode == "hydro":
+        ob.init_hydrostatic(seed, amp)
+    else:
+        ob.initialization()
+    return ob
+
+
+def gpu_box(ob, precision=E.FP64, init=True):
+    p = ob.p
+    params = E.make_params(p.nx, p.ny, p.nz, **ob.groups, lam=p.V, lam_c=p.VC, uw=p.uw, exf=p.exf,
+                           precision=precision, init_mode=E.INIT_FIELDS)
+    b = E.Box(params)
+    if init:
+        b.init(np.stack([ob[k] for k in FIELDS]))
+    return b
+
+
+def oracle_pops(ob):
+    return ob[O.F0], ob[O.F1], ob[O.H0], ob[O.H1]
+
+
+def assert_same(got, ref, what):
+    if not np.array_equal(got, ref):
+        diff = np.abs(got - ref)
+        idx = np.unravel_index(np.argmax(diff), diff.shape)
+        raise AssertionError(f"{what}: not bit-identical, max abs {diff.max():.3e} at {idx} "
+                             f"({np.count_nonzero(diff)} of {diff.size} differ)")
+
+
+def compare_pops(b, ob, what):
+    f0, f1, h0, h1 = b.read_populations()
+    rf0, rf1, rh0, rh1 = oracle_pops(ob)
+    assert_same(f1, rf1, what + " f1")
+    assert_same(h1, rh1, what + " h1")
+    assert_same(f0, rf0, what + " f0")
+    assert_same(h0, rh0, what + " h0")
+
+
+def compare_macros(b, ob, what):
+    g = b.read_fields(("rho", "q", "ux", "uy", "uz"))
+    for name, key in (("rho", O.RHO), ("q", O.Q), ("ux", O.UX), ("uy", O.UY), ("uz", O.UZ)):
+        assert_same(g[name], ob[key], f"{what} {name}")
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("shape", [(8, 6, 7), (16, 12, 10), (64, 64, 32)])
+def test_init_equilibrium_bitexact(shape):
+    ob = oracle_box(*shape)
+    ob.init_equilibrium()
+    with gpu_box(ob) as b:
+        compare_pops(b, ob, "init")
+
+
+def test_product_hydrostatic_init_matches_oracle():
+    nx, ny, nz = 16, 12, 10
+    ob = oracle_box(nx, ny, nz, seed=5, amp=1e-3)
+    ob.init_equilibrium()
+    p = E.make_params(nx, ny, nz, T=200.0, C_=10.0, lam=0.25, precision=E.FP64,
+                      init_mode=E.INIT_HYDROSTATIC, seed=5, amp=1e-3)
+    with E.Box(p) as b:
+        b.init()
+        f = b.read_fields()
+        for name, key in (("rho", O.RHO), ("q", O.Q), ("phi", O.PHI), ("ux", O.UX), ("uy", O.UY),
+                          ("uz", O.UZ), ("ex", O.EX), ("ey", O.EY), ("ez", O.EZ)):
+            assert np.allclose(f[name], ob[key], rtol=1e-14, atol=1e-14 * max(1.0, np.abs(ob[key]).max())), name
+        compare_pops(b, ob, "hydrostatic init")
+
+
+@pytest.mark.parametrize("case", [
+    dict(shape=(8, 6, 7)),
+    dict(shape=(16, 12, 10), lam=1.0 / 12.0),
+    dict(shape=(12, 8, 9), uw=0.05, exf=40.0),
+    dict(shape=(64, 64, 32)),
+])
+def test_lbm_steps_bitexact(case):
+    """Three LBM steps (no Poisson in between: E stays the initial field)."""
+    ob = oracle_box(*case["shape"], lam=case.get("lam", 0.25), uw=case.get("uw", 0.0), exf=case.get("exf", 0.0))
+    ob.init_equilibrium()
+    with gpu_box(ob) as b:
+        for k in range(3):
+            b.lbm_step()
+            ob.stream_collide_save()
+            compare_pops(b, ob, f"step {k + 1}")
+            compare_macros(b, ob, f"step {k + 1}")
+
+
+def test_reference_initial_state_bitexact():
+    """gpu_initialization state (q=0, linear phi, E=0) through two LBM steps."""
+    ob = oracle_box(12, 10, 8, mode="reference")
+    ob.init_equilibrium()
+    p = E.make_params(12, 10, 8, **ob.groups, lam=0.25, precision=E.FP64, init_mode=E.INIT_REFERENCE)
+    with E.Box(p) as b:
+        b.init()
+        compare_pops(b, ob, "reference init")
+        for k in range(2):
+            b.lbm_step()
+            ob.stream_collide_save()
+            compare_pops(b, ob, f"reference step {k + 1}")
+
+
+def test_lbm_with_phi_stencil_bitexact():
+    """E = -grad(phi) computed inside the fused kernel equals efield() + collide."""
+    ob = oracle_box(16, 12, 10)
+    ob.init_equilibrium()
+    with gpu_box(ob) as b:
+        b.lbm_step()
+        ob.stream_collide_save()
+        ob.poisson()
+        b.write_phi(ob[O.PHI])
+        ob.efield()
+        e = b.read_fields(("ex", "ey", "ez"))
+        assert_same(e["ex"], ob[O.EX], "ex")
+        assert_same(e["ey"], ob[O.EY], "ey")
+        assert_same(e["ez"], ob[O.EZ], "ez")
+        for k in range(2):
+            b.lbm_step()
+            ob.stream_collide_save()
+            compare_pops(b, ob, f"phi step {k + 1}")
+            compare_macros(b, ob, f"phi step {k + 1}")
+
+
+def test_random_population_state_bitexact():
+    """Arbitrary (non-equilibrium) populations written through the C-ABI."""
+    ob = oracle_box(10, 8, 9)
+    rng = np.random.default_rng(7)
+    w = O.tables()["w"]
+    shp = ob.shape
+    ob[O.F0][...] = w[0] * 1600 * (1 + 1e-3 * rng.standard_normal(shp))
+    ob[O.H0][...] = w[0] * 5 * (1 + 0.1 * rng.standard_normal(shp))
+    for d in range(1, 27):
+        ob[O.F1][d - 1] = w[d] * 1600 * (1 + 1e-3 * rng.standard_normal(shp))
+        ob[O.H1][d - 1] = w[d] * 5 * (1 + 0.1 * rng.standard_normal(shp))
+    with gpu_box(ob) as b:
+        b.write_populations(*oracle_pops(ob))
+        compare_pops(b, ob, "written")
+        for k in range(2):
+            b.lbm_step()
+            ob.stream_collide_save()
+            compare_pops(b, ob, f"random step {k + 1}")
+
+
+@pytest.mark.parametrize("shape", [(8, 6, 7), (48, 48, 32), (80, 80, 12), (64, 64, 32), (122, 122, 10), (96, 96, 8)])
+@pytest.mark.parametrize("precision", [E.FP64, E.FP32S])
+def test_poisson_vs_oracle(shape, precision):
+    nx, ny, nz = shape
+    ob = oracle_box(nx, ny, nz)
+    rng = np.random.default_rng(nx + nz)
+    q = ob[O.Q] * (1 + 0.2 * rng.standard_normal(ob.shape))
+    ob[O.Q][...] = q
+    with gpu_box(ob, precision=precision) as b:
+        b.write_charge(q)
+        b.poisson()
+        phi = b.read_fields(("phi",))["phi"]
+    ob.poisson()
+    err = np.max(np.abs(phi - ob[O.PHI])) / np.max(np.abs(ob[O.PHI]))
+    tol = 1e-13 if precision == E.FP64 else 2e-6
+    print(f"poisson {shape} prec={precision}: max rel err {err:.3e}")
+    assert err < tol
+
+
+def test_diagnostics_current_vs_oracle():
+    ob = oracle_box(16, 12, 10)
+    ob.init_equilibrium()
+    with gpu_box(ob) as b:
+        b.step(3)
+        ob.step(3)
+        d = b.diagnostics()
+        ref = ob.current()
+        assert abs(d.current / ref - 1) < 1e-11, (d.current, ref)
+        q = ob[O.Q][1:-1]
+        assert abs(d.total_charge / (q.sum() * ob.p.dx ** 3) - 1) < 1e-12
+        uz = ob[O.UZ].copy()
+        uz[-1] = 2 * uz[-2] - uz[-3]
+        assert d.umax == max(0.0, uz.max())
+        assert d.step == 3
+
+
+@pytest.mark.parametrize("nsteps", [200])
+def test_coupled_fp64_config1_vs_oracle(nsteps):
+    """64x64x32, T=200, C=10, M=10, Fe=4000, Lambda=1/4, hydrostatic start + 1e-3 noise."""
+    ob = oracle_box(64, 64, 32)
+    ob.init_equilibrium()
+    with gpu_box(ob) as b:
+        b.step(nsteps)
+        g = b.read_fields(("q", "phi", "ux", "uy", "uz"))
+    ob.step(nsteps)
+    errs = {
+        "u": rel_l2(np.stack([g["ux"], g["uy"], g["uz"]])[:, 1:-1],
+                    np.stack([ob[O.UX], ob[O.UY], ob[O.UZ]])[:, 1:-1]),
+        "q": rel_l2(g["q"][1:-1], ob[O.Q][1:-1]),
+        "phi": rel_l2(g["phi"], ob[O.PHI]),
+    }
+    print(f"coupled FP64 {nsteps} steps rel-L2: {errs}")
+    for k, v in errs.items():
+        assert v < 1e-12, (k, v)
+
+
+def test_fp32s_vs_fp64_short():
+    """FP32 storage of shifted populations against the FP64 path, same start."""
+    ob = oracle_box(32, 32, 16)
+    ob.init_equilibrium()
+    res = {}
+    for prec in (E.FP64, E.FP32S):
+        with gpu_box(ob, precision=prec) as b:
+            b.step(300)
+            res[prec] = b.read_fields(("q", "phi", "ux", "uy", "uz"))
+    a, r = res[E.FP32S], res[E.FP64]
+    errs = {k: rel_l2(a[k][1:-1], r[k][1:-1]) for k in ("q", "phi", "ux", "uy", "uz")}
+    print(f"FP32S vs FP64 after 300 steps: {errs}")
+    assert errs["q"] < 1e-5 and errs["phi"] < 1e-5
+    assert max(errs["ux"], errs["uy"], errs["uz"]) < 1e-2
