@@ -1,0 +1,139 @@
+"""Coupled EHD step benchmark on one MI355X.
+
+Workload (the single-GPU configuration the headline metric is quoted on):
+256 x 256 x 128 box, FP32-shifted populations, T=400, C=10, M=10, Fe=4000,
+Pr=10, TRT Lambda=1/4, hydrostatic start plus seeded 1e-3 velocity noise.
+A "step" is one full coupled step: fused LBM collide/stream/boundary for f
+and h, then the Poisson solve (x,y FFTs, per-mode tridiagonal z solve,
+inverse FFTs).  MLUPS = nx*ny*nz*steps / (1e6 * seconds), walls included
+(main.cu:256-258).
+
+Prints one JSON line.  roofline: the fused LBM kernel (the dominant kernel),
+algorithmic bytes 440 B per node (2 x 27 populations read and written in
+FP32, q written, phi read) over its average launch time, taken from HIP
+events recorded on the box's stream around every launch inside the timed
+region.  cpu_baseline: the FP64 CPU oracle (OpenMP) on a bounded sample of
+the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "ehd-convection-3d_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+LBM_BYTES_PER_NODE = {1: 440.0, 0: 880.0}     # FP32S, FP64
+STEP_BYTES_PER_NODE = {1: 448.0, 0: 896.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ny", type=int, default=256)
+    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--precision", choices=("fp32s", "fp64"), default="fp32s")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps in the CPU sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle OpenMP threads (0: all host cores)")
+    ap.add_argument("--no-profile", action="store_true", help="no per-launch events (value only)")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(workload: str):
+    """HBM bytes per LBM launch measured with rocprofv3 PMC counters (profiles/)."""
+    path = os.path.join(ROOT, "profiles", "lbm_traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data.get(workload, {}).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(args, workload):
+    from oracle import pyoracle as O
+
+    threads = args.cpu_threads or (os.cpu_count() or 1)
+    O.set_threads(threads)
+    p = O.make_params(args.nx, args.ny, args.nz, T=400.0, M=10.0, C_=10.0, Fe=4000.0, lam=0.25)
+    b = O.Box(p)
+    b.init_hydrostatic(2, 1e-3)
+    b.init_equilibrium()
+    b.step(1)                       # warm-up step
+    n = args.cpu_steps
+    t0 = time.perf_counter()
+    b.step(n)
+    dt = time.perf_counter() - t0
+    S = args.nx * args.ny * args.nz
+    return {"value": round(S * n / dt / 1e6, 4), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, {workload}, {n} coupled steps after 1 warm-up step, {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the coupled step is "
+              "single-GPU in this build", file=sys.stderr)
+        sys.exit(2)
+    import ehd_amd as E
+
+    prec = E.FP32S if args.precision == "fp32s" else E.FP64
+    nx, ny, nz = args.nx, args.ny, args.nz
+    S = nx * ny * nz
+    workload = f"coupled EHD step {nx}x{ny}x{nz} {args.precision}"
+    params = E.make_params(nx, ny, nz, T=400.0, C_=10.0, M=10.0, Fe=4000.0, Pr=10.0, lam=0.25,
+                           precision=prec, init_mode=E.INIT_HYDROSTATIC, seed=2, amp=1e-3)
+    box = E.Box(params)
+    box.init()
+    box.step(args.warmup)
+    box.sync()
+    if not args.no_profile:
+        box.profile_enable(True)
+    box.profile_reset()
+    t0 = time.perf_counter()
+    box.step(args.steps)
+    box.sync()
+    elapsed = time.perf_counter() - t0
+    prof = box.profile_read()
+    d = box.diagnostics()
+    box.close()
+
+    mlups = S * args.steps / elapsed / 1e6
+    ms_step = elapsed * 1e3 / args.steps
+    line = {
+        "metric": "coupled-step MLUPS", "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if prec == E.FP32S else "f64", "data": "synthetic",
+        "config": {"workload": workload, "T": 400, "C": 10, "M": 10, "Fe": 4000, "Pr": 10,
+                   "lambda": 0.25, "init": "hydrostatic + 1e-3 noise, seed 2"},
+        "hbm_frac_step": round(STEP_BYTES_PER_NODE[prec] * S * args.steps / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    if not args.no_profile and prof.lbm_launches > 0:
+        lbm_ms = prof.lbm_ms / prof.lbm_launches
+        achieved = LBM_BYTES_PER_NODE[prec] * S / (lbm_ms * 1e-3) / 1e9
+        line["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(achieved / HBM_PEAK_GBS, 4),
+                            "traffic": traffic_from_profiles(workload),
+                            "kernel": "lbm_f32s_kernel" if prec == E.FP32S else "lbm_f64_kernel",
+                            "avg_launch_ms": round(lbm_ms, 4)}
+        line["kernel_ms"] = {"lbm": round(lbm_ms, 4),
+                             "poisson": round(prof.poisson_ms / max(prof.poisson_calls, 1), 4)}
+    line["diag"] = {"current": d.current, "umax": d.umax}
+    if args.cpu_steps > 0:
+        line["cpu_baseline"] = cpu_baseline(args, workload)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -60,12 +60,13 @@ struct ehd_box {
     double* phibar = nullptr;
     double *ex = nullptr, *ey = nullptr, *ez = nullptr;
     void* spec = nullptr;
-    double* dscr = nullptr;
-    double* cpt = nullptr;
     void* twx = nullptr;
     void* twy = nullptr;
     double* diag_scr = nullptr;
     double* diag_out = nullptr;
+    double* shift_rs = nullptr;   // FP32S storage shifts (device)
+    double* shift_qs = nullptr;
+    float* shift_f = nullptr;
     long long step = 0;
     double t = 0.0;
     double current_hs = 0.0;
@@ -155,7 +156,7 @@ void fill_lbm_params(ehd_box* b)
         L.inj[c] = 2.0 * d.charge0 * L.w[c];
     }
     L.dx = d.dx; L.dy = d.dy; L.dz = d.dz;
-    L.q_ref = d.charge0;
+    L.rs = nullptr; L.qs = nullptr; L.shf = nullptr; L.qs0 = 0.0;
 }
 
 template <typename T>
@@ -182,7 +183,8 @@ void free_box(ehd_box* b)
 {
     void* ptrs[] = {b->f[0], b->f[1], b->f0[0], b->f0[1], b->h[0], b->h[1], b->h0[0], b->h0[1],
                     b->rho, b->ux, b->uy, b->uz, b->q, b->phi, b->phibar, b->ex, b->ey, b->ez,
-                    b->spec, b->dscr, b->cpt, b->twx, b->twy, b->diag_scr, b->diag_out};
+                    b->spec, b->twx, b->twy, b->diag_scr, b->diag_out,
+                    b->shift_rs, b->shift_qs, b->shift_f};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& pr : b->lbm_pairs) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
@@ -234,7 +236,7 @@ int run_poisson(ehd_box* b)
         if (!e0 || !e1) return fail(EHD_EHIP, "hipEventCreate failed");
         HIPCHK(hipEventRecord(e0, b->st));
     }
-    PoissonBufs pb{b->nx, b->ny, b->nz, b->nxh, b->spec, b->dscr, b->cpt, b->twx, b->twy, b->phibar};
+    PoissonBufs pb{b->nx, b->ny, b->nz, b->nxh, b->spec, b->twx, b->twy, b->phibar};
     hipError_t e = launch_poisson(b->prec, b->lp, pb, b->q, b->phi, b->prm.voltage, b->prm.eps, b->st);
     if (e != hipSuccess) return fail(EHD_EHIP, "Poisson launch failed: %s", hipGetErrorString(e));
     if (b->prof) {
@@ -363,35 +365,49 @@ int ehd_create(const ehd_params* p, ehd_box** out)
         (rc = alloc((void**)&b->ey, sizeof(double) * S)) || (rc = alloc((void**)&b->ez, sizeof(double) * S)))
         return bail(rc);
     const size_t nmodes = (size_t)b->nxh * b->ny, nint = (size_t)(b->nz - 2);
-    if ((rc = alloc(&b->spec, nmodes * nint * 2 * es)) || (rc = alloc((void**)&b->dscr, nmodes * nint * 2 * sizeof(double))) ||
-        (rc = alloc((void**)&b->cpt, nmodes * nint * sizeof(double))) || (rc = alloc(&b->twx, 2 * (size_t)b->nx * es)) ||
+    if ((rc = alloc(&b->spec, nmodes * nint * 2 * es)) || (rc = alloc(&b->twx, 2 * (size_t)b->nx * es)) ||
         (rc = alloc(&b->twy, 2 * (size_t)b->ny * es)) ||
         (rc = alloc((void**)&b->diag_scr, sizeof(double) * 8 * (size_t)diag_blocks())) ||
         (rc = alloc((void**)&b->diag_out, sizeof(double) * 8)))
         return bail(rc);
 
-    // Thomas pivots of each (kx,ky) mode: b = -(2 + k^2 dz^2), c'_0 = 1/b, c'_k = 1/(b - c'_{k-1}).
-    // Wavenumbers as main.cu:109-125.
-    {
-        std::vector<double> cp(nmodes * nint);
-        const double dz2 = b->der.dz * b->der.dz;
-        for (int ky = 0; ky < b->ny; ++ky) {
-            const double kyv = (ky <= b->ny / 2) ? (double)ky * 2.0 * kPi / b->der.ly
-                                                 : ((double)ky - b->ny) * 2.0 * kPi / b->der.ly;
-            for (int kx = 0; kx < b->nxh; ++kx) {
-                const double kxv = (double)kx * 2.0 * kPi / b->der.lx;
-                const double bd = -(2.0 + (kxv * kxv + kyv * kyv) * dz2);
-                const size_t m = (size_t)ky * b->nxh + kx;
-                double c = 1.0 / bd;
-                cp[m] = c;
-                for (size_t k = 1; k < nint; ++k) {
-                    c = 1.0 / (bd - c);
-                    cp[k * nmodes + m] = c;
-                }
-            }
+    if (b->prec == Prec::F32S) {
+        // Storage shifts: the hydrostatic state of unipolar injection, q(z) of
+        // the analytic profile and rho(z) = rho0 + (p(z) - mean p)/cs^2 with
+        // dp/dz = q Ez = const (the uniform current density).
+        const int nz = b->nz;
+        const double C = b->der.charge0 * p->lz * p->lz / (p->voltage * p->eps);
+        const double e0 = hydrostatic_e0(C);
+        const double qsc = p->eps * p->voltage / (p->lz * p->lz);
+        const double J = C * e0 * p->eps * p->voltage * p->voltage / (p->lz * p->lz * p->lz);
+        std::vector<double> rs(nz), qs(nz);
+        for (int z = 0; z < nz; ++z) {
+            const double zeta = b->der.dz * z / p->lz;
+            qs[z] = C * e0 / std::sqrt(e0 * e0 + 2.0 * C * e0 * zeta) * qsc;
+            rs[z] = p->rho0 + J * (b->der.dz * z - 0.5 * p->lz) / b->der.cs_square;
         }
-        if (hipMemcpy(b->cpt, cp.data(), cp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-            return bail(fail(EHD_EHIP, "pivot upload failed"));
+        qs[nz - 1] = qs[nz - 2];   // the top plane holds copies of plane nz-2
+        std::vector<float> tf(6 * (size_t)nz);
+        for (int z = 0; z < nz; ++z) {
+            const int zm = (z == 0) ? nz - 1 : z - 1, zp = (z == nz - 1) ? 0 : z + 1;
+            tf[z] = (float)rs[z];
+            tf[nz + z] = (float)(rs[zm] - rs[z]);
+            tf[2 * nz + z] = (float)(rs[zp] - rs[z]);
+            tf[3 * nz + z] = (float)qs[z];
+            tf[4 * nz + z] = (float)(qs[zm] - qs[z]);
+            tf[5 * nz + z] = (float)(qs[zp] - qs[z]);
+        }
+        if ((rc = alloc((void**)&b->shift_rs, nz * sizeof(double))) || (rc = alloc((void**)&b->shift_qs, nz * sizeof(double))) ||
+            (rc = alloc((void**)&b->shift_f, tf.size() * sizeof(float))))
+            return bail(rc);
+        if (hipMemcpy(b->shift_rs, rs.data(), nz * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->shift_qs, qs.data(), nz * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->shift_f, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(EHD_EHIP, "shift table upload failed"));
+        b->lp.rs = b->shift_rs;
+        b->lp.qs = b->shift_qs;
+        b->lp.shf = b->shift_f;
+        b->lp.qs0 = qs[0];
     }
     if (b->prec == Prec::F64) {
         std::vector<double> tx, ty;
@@ -437,16 +453,19 @@ int ehd_init(ehd_box* b, const double* fields9)
     } guard{tmp};
     double* dqz = nullptr;
     if (p.init_mode == EHD_INIT_REFERENCE) {
-        HIPCHK(launch_init_fields(b->lp, 0, nullptr, nullptr, 0, 0.0, p.voltage, p.lz, rho, q, phi, ux, uy, uz, b->st));
+        HIPCHK(launch_init_fields(b->lp, 0, nullptr, nullptr, nullptr, 0, 0.0, p.voltage, p.lz, rho, q, phi, ux, uy, uz,
+                                  b->st));
         HIPCHK(hipMemsetAsync(b->ex, 0, S * sizeof(double), b->st));
         HIPCHK(hipMemsetAsync(b->ey, 0, S * sizeof(double), b->st));
         HIPCHK(hipMemsetAsync(b->ez, 0, S * sizeof(double), b->st));
     } else if (p.init_mode == EHD_INIT_HYDROSTATIC) {
         const int nz = b->nz;
-        std::vector<double> tab(2 * (size_t)nz);
+        std::vector<double> tab(3 * (size_t)nz);
         const double C = b->der.charge0 * p.lz * p.lz / (p.voltage * p.eps);
         const double e0 = hydrostatic_e0(C);
         const double qs = p.eps * p.voltage / (p.lz * p.lz);
+        // uniform current density J = q Ez balances the pressure: dp/dz = J
+        const double J = C * e0 * p.eps * p.voltage * p.voltage / (p.lz * p.lz * p.lz);
         for (int z = 0; z < nz; ++z) {
             const double zeta = b->der.dz * z / p.lz;
             const double e = std::sqrt(e0 * e0 + 2.0 * C * e0 * zeta);
@@ -455,12 +474,13 @@ int ehd_init(ehd_box* b, const double* fields9)
             if (z == 0) ph = p.voltage;
             if (z == nz - 1) ph = 0.0;
             tab[nz + z] = ph;
+            tab[2 * nz + z] = p.hydro_density ? p.rho0 + J * (b->der.dz * z - 0.5 * p.lz) / b->der.cs_square : p.rho0;
         }
         rc = alloc((void**)&dqz, tab.size() * sizeof(double));
         if (rc) return rc;
         hipError_t e1 = hipMemcpyAsync(dqz, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, b->st);
-        hipError_t e2 = launch_init_fields(b->lp, 1, dqz, dqz + nz, p.seed, p.perturb_amp, p.voltage, p.lz, rho, q, phi,
-                                           ux, uy, uz, b->st);
+        hipError_t e2 = launch_init_fields(b->lp, 1, dqz, dqz + nz, dqz + 2 * nz, p.seed, p.perturb_amp, p.voltage,
+                                           p.lz, rho, q, phi, ux, uy, uz, b->st);
         hipError_t e3 = launch_efield(Prec::F64, b->lp, phi, nullptr, b->ex, b->ey, b->ez, b->st);
         hipError_t e4 = hipStreamSynchronize(b->st);
         (void)hipFree(dqz);

@@ -20,15 +20,19 @@ struct LbmParams {
     double multi[4];             // moving-wall terms per weight class (LBM.cu:1137-1139)
     double inj[4];               // 2*charge0*w (LBM.cu:1288-1291)
     double dx, dy, dz;
-    double q_ref;                // FP32S charge shift
+    // FP32S storage shifts (nullptr for F64): f is stored as f - w*rs[z], h as
+    // h - w*qs[z]; shf packs the float tables rs, rs[zm]-rs, rs[zp]-rs, qs,
+    // qs[zm]-qs, qs[zp]-qs, nz entries each (zm, zp: periodic neighbours).
+    const double* rs;
+    const double* qs;
+    const float* shf;
+    double qs0;                  // host copy of qs[0]
 };
 
 // Device buffers of the Poisson solve.
 struct PoissonBufs {
     int nx, ny, nz, nxh;         // nxh = nx/2+1
     void* spec;                  // complex [nz-2][ny][nxh], element type of the path
-    double* dscr;                // FP64 complex forward-sweep values [nz-2][ny][nxh]
-    const double* cpt;           // Thomas pivots 1/(b - c'), [nz-2][ny][nxh]
     const void* twx;             // nx roots exp(-2 pi i j/nx), element type of the path
     const void* twy;             // ny roots
     double* phibar;              // FP32S: xy-mean potential per plane [nz] (FP64)
@@ -83,7 +87,7 @@ hipError_t launch_phi_split(Prec prec, const LbmParams& P, const double* phi, vo
 
 // Reference and hydrostatic initial fields (doubles).
 hipError_t launch_init_fields(const LbmParams& P, int mode, const double* qz, const double* phiz,
-                              uint64_t seed, double amp, double voltage, double lz, double* rho,
+                              const double* rhoz, uint64_t seed, double amp, double voltage, double lz, double* rho,
                               double* q, double* phi, double* ux, double* uy, double* uz,
                               hipStream_t st);
 int diag_blocks();

@@ -31,21 +31,23 @@ __device__ __forceinline__ double hash_uniform(uint64_t seed, uint32_t x, uint32
 // mode 0: gpu_initialization (LBM.cu:80-96); mode 1: hydrostatic profile
 // (tables qz, phiz per plane) plus the seeded perturbation of u.
 __global__ void __launch_bounds__(256) init_fields_kernel(LbmParams P, int mode, const double* qz,
-                                                          const double* phiz, uint64_t seed, double amp,
+                                                          const double* phiz, const double* rhoz, uint64_t seed,
+                                                          double amp,
                                                           double voltage, double lz, double* rho, double* q,
                                                           double* phi, double* ux, double* uy, double* uz)
 {
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= P.S) return;
     const unsigned x = s % P.nx, t = s / P.nx, y = t % P.ny, z = t / P.ny;
-    rho[s] = P.rho0;
     if (mode == 0) {
+        rho[s] = P.rho0;
         q[s] = 0.0;
         phi[s] = voltage * (lz - P.dz * z) / lz;
         ux[s] = 0.0;
         uy[s] = 0.0;
         uz[s] = 0.0;
     } else {
+        rho[s] = rhoz[z];
         q[s] = qz[z];
         phi[s] = phiz[z];
         const bool wall = (z == 0) || (z == (unsigned)P.nz - 1);
@@ -205,12 +207,12 @@ __global__ void __launch_bounds__(256) diag_final_kernel(const double* part, int
 
 }  // namespace
 
-hipError_t launch_init_fields(const LbmParams& P, int mode, const double* qz, const double* phiz, uint64_t seed,
-                               double amp, double voltage, double lz, double* rho, double* q, double* phi,
-                               double* ux, double* uy, double* uz, hipStream_t st)
+hipError_t launch_init_fields(const LbmParams& P, int mode, const double* qz, const double* phiz, const double* rhoz,
+                              uint64_t seed, double amp, double voltage, double lz, double* rho, double* q, double* phi,
+                              double* ux, double* uy, double* uz, hipStream_t st)
 {
-    init_fields_kernel<<<grid_for(P.S), 256, 0, st>>>(P, mode, qz, phiz, seed, amp, voltage, lz, rho, q, phi, ux,
-                                                      uy, uz);
+    init_fields_kernel<<<grid_for(P.S), 256, 0, st>>>(P, mode, qz, phiz, rhoz, seed, amp, voltage, lz, rho, q, phi,
+                                                      ux, uy, uz);
     return hipGetLastError();
 }
 

@@ -1,21 +1,28 @@
 // lbm_f32s.hip -- fused pull-scheme LBM step with FP32 storage of shifted
 // populations (the performance path).
 //
-// Storage: g_d = f_d - w_d*rho0 and k_d = h_d - w_d*q_ref (q_ref = charge0),
-// float, SoA [d-1][z][y][x], double buffered; same wall and representation
-// rules as the FP64 kernel (lbm_f64.hip), which restates the reference step
-// (LBM.cu:278-1353) in its own operation order.  Here the collision is
-// written directly on the deviations, pair by pair (d, d+1 are opposites):
-//   rho = rho0 + sum g,  q = q_ref + sum k,
+// Storage: g_d = f_d - w_d*rs(z) and k_d = h_d - w_d*qs(z), float, SoA
+// [d-1][z][y][x], double buffered, where rs(z), qs(z) are the hydrostatic
+// density and charge profiles of the box (fixed at creation).  The shifts take
+// the large, smooth part of every population out of the float rounding: what
+// is stored is the deviation from the hydrostatic state, so the rounding of
+// a stored population is relative to that deviation, not to w*rho0 or to the
+// hydrostatic pressure or charge.  A population pulled from plane z' is
+// re-shifted to plane z by adding w_d (rs(z') - rs(z)).
+//
+// Same wall and representation rules as the FP64 kernel (lbm_f64.hip), which
+// restates the reference step (LBM.cu:278-1353) in its own operation order.
+// Here the collision is written directly on the deviations, pair by pair
+// (d, d+1 are opposites):
+//   rho = rs + sum g,  q = qs + sum k,
 //   j_a = sum_d c_da g_d,  u = (j/CFL + F dt/2)/rho,  F = q E (+ exf x),
 //   t_d = c_d.u /(CFL cs^2),
-//   geq+ = w (rho - rho0 + rho (t^2/2 - u^2/(2 cs^2))),  geq- = w rho t,
-//   F+  = w (c.u c.F /(CFL^2 cs^4) - u.F/cs^2),      F-  = w c.F /(CFL cs^2),
+//   geq+ = w (rho - rs + rho (t^2/2 - u^2/(2 cs^2))),  geq- = w rho t,
+//   F+  = w (c.u c.F /(CFL^2 cs^4) - u.F/cs^2),       F-  = w c.F /(CFL cs^2),
 //   g'_d = g_d - om+ (g+ - geq+) - om- (g- - geq-) + dt (s+ F+ + s- F-),
 // which is the algebra of gpu_collide_save (LBM.cu:519-1089) with the shift
 // cancelling in every difference (w_d = w_opp(d), sum_d w_d = 1).  h uses
-// u + K E, no source, and the charge rates.  Arithmetic is FP32; the large
-// parts (w*rho0, w*q_ref, the wall potential) never enter a float rounding.
+// u + K E, no source, and the charge rates.  Arithmetic is FP32.
 #include <hip/hip_runtime.h>
 
 #include "ehd_internal.h"
@@ -28,14 +35,22 @@ struct F32Consts {
     float inv2cs2;            // 1/(2 cs^2)
     float B;                  // 1/(CFL^2 cs^4)
     float ics2;               // 1/cs^2
-    float K, exf, rho0, qref;
+    float K, exf;
     float omp, omm, ocp, ocm; // omega*dt
     float sp, sm;             // 1 - omega dt/2
     float w[4];
     float multi[4];
-    float injk[4];            // 2 w (charge0 - q_ref)
+    float injk[4];            // 2 w (charge0 - qs(0))
     float hdx, hdy, hdz;      // 0.5/dx ...
 };
+
+// shift correction of a population of direction d pulled from plane z - c_z
+__device__ __forceinline__ float pulled(float v, int d, float w, float dup, float ddn)
+{
+    if (CZ[d] > 0) return v + w * dup;
+    if (CZ[d] < 0) return v + w * ddn;
+    return v;
+}
 
 template <bool FIRST, bool EBUF>
 __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c, PopBufs pb, MacroBufs mb,
@@ -57,6 +72,10 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
     const unsigned YS[3] = {ym * nx, y * nx, yp * nx};
     const unsigned ZS[3] = {zm * nxy, z * nxy, zp * nxy};
 
+    const float* __restrict__ sh = P.shf;
+    const float rs = sh[z], dRu = sh[nz + z], dRd = sh[2 * nz + z];
+    const float qs = sh[3 * nz + z], dQu = sh[4 * nz + z], dQd = sh[5 * nz + z];
+
     const float* __restrict__ fA = (const float*)pb.f;
     const float* __restrict__ f0A = (const float*)pb.f0;
     const float* __restrict__ hA = (const float*)pb.h;
@@ -73,7 +92,8 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
         }
     } else {
 #pragma unroll
-        for (int d = 1; d < 27; ++d) g[d] = fA[(d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]];
+        for (int d = 1; d < 27; ++d)
+            g[d] = pulled(fA[(d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]], d, c.w[WCLS[d]], dRu, dRd);
         if (z == 0) {
             k[0] = -h0A[s] + c.injk[0];
 #pragma unroll
@@ -81,7 +101,9 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
         } else {
             k[0] = h0A[s];
 #pragma unroll
-            for (int d = 1; d < 27; ++d) k[d] = hA[(d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]];
+            for (int d = 1; d < 27; ++d)
+                k[d] = pulled(hA[(d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]], d, c.w[WCLS[d]], dQu,
+                              dQd);
         }
     }
 
@@ -106,8 +128,8 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
         drho += g[d] + g[d + 1];
         dq += k[d] + k[d + 1];
     }
-    const float rho = c.rho0 + drho;
-    const float charge = c.qref + dq;
+    const float rho = rs + drho;
+    const float charge = qs + dq;
     float j[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -129,9 +151,8 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
     if (z == 0) {
         // velocity override from the pre-collision populations of z=1 (LBM.cu:416-493)
         float gm[27];
-        float dqm;
         const unsigned s1 = s + nxy;
-        dqm = h0A[s1];
+        float dqm = h0A[s1];
         if (FIRST) {
 #pragma unroll
             for (int d = 1; d < 27; ++d) {
@@ -140,11 +161,12 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
             }
         } else {
             const unsigned Z1[3] = {0u, nxy, 2u * nxy};
+            const float dRu1 = sh[nz + 1], dRd1 = sh[2 * nz + 1], dQu1 = sh[4 * nz + 1], dQd1 = sh[5 * nz + 1];
 #pragma unroll
             for (int d = 1; d < 27; ++d) {
                 const unsigned src = XS[1 - CX[d]] + YS[1 - CY[d]] + Z1[1 - CZ[d]];
-                gm[d] = fA[(d - 1) * S + src];
-                dqm += hA[(d - 1) * S + src];
+                gm[d] = pulled(fA[(d - 1) * S + src], d, c.w[WCLS[d]], dRu1, dRd1);
+                dqm += pulled(hA[(d - 1) * S + src], d, c.w[WCLS[d]], dQu1, dQd1);
             }
         }
         float Exm, Eym, Ezm;
@@ -156,7 +178,7 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
             const float dmn = (float)(mb.phibar[0] - mb.phibar[2]);
             Ezm = c.hdz * ((phi[x + yc] - phi[x + yc + 2 * nxy]) + dmn);
         }
-        const float qm = c.qref + dqm;
+        const float qm = sh[3 * nz + 1] + dqm;
         float jm[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -202,7 +224,7 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
         f0B[s] = fwall ? g[0] : g0n;
         if (z != nz - 1) {
             h0B[s] = k0n;
-            if (z == nz - 2) h0B[s + nxy] = k0n;
+            if (z == nz - 2) h0B[s + nxy] = k0n;   // qs(nz-1) = qs(nz-2): no re-shift
         }
     }
 #pragma unroll
@@ -228,7 +250,7 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
         const float ge = g[e] - rp + rm + sP - sM;
         const float kd = k[d] - qp - qm2;
         const float ke = k[e] - qp + qm2;
-        // f: bounce-back of the pre-collision values at the walls
+        // f: bounce-back of the pre-collision values at the walls (same plane, same shift)
         float od, oe;
         if (z == 0) {
             od = g[e];
@@ -268,8 +290,6 @@ hipError_t launch_lbm_f32s(const LbmParams& P, const PopBufs& pb, const MacroBuf
     c.ics2 = (float)(1.0 / cs2);
     c.K = (float)P.K;
     c.exf = (float)P.exf;
-    c.rho0 = (float)P.rho0;
-    c.qref = (float)P.q_ref;
     c.omp = (float)(P.omega_plus * P.dt);
     c.omm = (float)(P.omega_minus * P.dt);
     c.ocp = (float)(P.omega_c_plus * P.dt);
@@ -279,7 +299,7 @@ hipError_t launch_lbm_f32s(const LbmParams& P, const PopBufs& pb, const MacroBuf
     for (int i = 0; i < 4; ++i) {
         c.w[i] = (float)P.w[i];
         c.multi[i] = (float)P.multi[i];
-        c.injk[i] = (float)(2.0 * P.w[i] * (P.charge0 - P.q_ref));
+        c.injk[i] = (float)(2.0 * P.w[i] * (P.charge0 - P.qs0));
     }
     c.hdx = (float)(0.5 / P.dx);
     c.hdy = (float)(0.5 / P.dy);

@@ -328,16 +328,17 @@ __global__ void __launch_bounds__(256) lbm_f64_kernel(LbmParams P, PopBufs pb, M
 }
 
 // Equilibrium populations (gpu_init_equilibrium, LBM.cu:114-276) in FP64,
-// stored as doubles (F64) or as float deviations from w*rho0, w*q_ref (F32S).
+// stored as doubles (F64) or as float deviations from w*rs[z], w*qs[z] (F32S).
 template <typename T>
 __global__ void __launch_bounds__(256) init_eq_kernel(LbmParams P, const double* rhoA, const double* qA,
                                                       const double* uxA, const double* uyA,
                                                       const double* uzA, const double* exA,
                                                       const double* eyA, const double* ezA, T* f, T* f0,
-                                                      T* h, T* h0, int shifted)
+                                                      T* h, T* h0)
 {
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= P.S) return;
+    const unsigned z = s / P.nxy;
     const double cs_square = P.cs_square, CFL = P.cfl, K = P.K;
     const double rho = rhoA[s], ux = uxA[s], uy = uyA[s], uz = uzA[s], charge = qA[s];
     const double Ex = exA[s], Ey = eyA[s], Ez = ezA[s];
@@ -354,7 +355,7 @@ __global__ void __launch_bounds__(256) init_eq_kernel(LbmParams P, const double*
     tc[1] = (ux + K * Ex) / cs_square / CFL;
     tc[2] = (uy + K * Ey) / cs_square / CFL;
     tc[3] = (uz + K * Ez) / cs_square / CFL;
-    const double rs = shifted ? P.rho0 : 0.0, qs = shifted ? P.q_ref : 0.0;
+    const double rs = P.rs ? P.rs[z] : 0.0, qs = P.qs ? P.qs[z] : 0.0;
     f0[s] = (T)(wr[0] * (omusq) - P.w[0] * rs);
     h0[s] = (T)(wc[0] * (omusq_c) - P.w[0] * qs);
     const unsigned S = P.S;
@@ -371,11 +372,12 @@ __global__ void __launch_bounds__(256) init_eq_kernel(LbmParams P, const double*
 }
 
 // Reference pre-collision state (f0, f1, h0, h1 after stream and bc_charge)
-// as doubles, from the box's representation.
+// as doubles, from the box's representation.  A stored value of plane z
+// carries the shift w*rs[z] (w*qs[z]); F64 has no shift.
 template <typename T>
-__global__ void __launch_bounds__(256) read_pre_kernel(LbmParams P, int direct, int shifted, const T* f,
-                                                       const T* f0, const T* h, const T* h0, double* f1o,
-                                                       double* f0o, double* h1o, double* h0o)
+__global__ void __launch_bounds__(256) read_pre_kernel(LbmParams P, int direct, const T* f, const T* f0,
+                                                       const T* h, const T* h0, double* f1o, double* f0o,
+                                                       double* h1o, double* h0o)
 {
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= P.S) return;
@@ -387,35 +389,35 @@ __global__ void __launch_bounds__(256) read_pre_kernel(LbmParams P, int direct, 
     const unsigned XS[3] = {xm, x, xp};
     const unsigned YS[3] = {ym * nx, y * nx, yp * nx};
     const unsigned ZS[3] = {zm * nxy, z * nxy, zp * nxy};
-    const double rs = shifted ? P.rho0 : 0.0, qs = shifted ? P.q_ref : 0.0;
-    if (f0o) f0o[s] = (double)f0[s] + P.w[0] * rs;
-    if (h0o) {
-        if (!direct && z == 0) h0o[s] = -((double)h0[s] + P.w[0] * qs) + P.inj[0];
-        else h0o[s] = (double)h0[s] + P.w[0] * qs;
-    }
+    const unsigned ZP[3] = {zm, z, zp};
+    auto rsh = [&](unsigned zz) { return P.rs ? P.rs[zz] : 0.0; };
+    auto qsh = [&](unsigned zz) { return P.qs ? P.qs[zz] : 0.0; };
+    f0o[s] = (double)f0[s] + P.w[0] * rsh(z);
+    if (!direct && z == 0) h0o[s] = -((double)h0[s] + P.w[0] * qsh(0)) + P.inj[0];
+    else h0o[s] = (double)h0[s] + P.w[0] * qsh(z);
 #pragma unroll
     for (int d = 1; d < 27; ++d) {
         const double wf = P.w[WCLS[d]];
         const unsigned src = direct ? s : (XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]);
-        if (f1o) f1o[(d - 1) * S + s] = (double)f[(d - 1) * S + src] + wf * rs;
-        if (h1o) {
-            if (!direct && z == 0)
-                h1o[(d - 1) * S + s] = -((double)h[(opp(d) - 1) * S + s] + wf * qs) + P.inj[WCLS[d]];
-            else
-                h1o[(d - 1) * S + s] = (double)h[(d - 1) * S + src] + wf * qs;
-        }
+        const unsigned zs = direct ? z : ZP[1 - CZ[d]];
+        f1o[(d - 1) * S + s] = (double)f[(d - 1) * S + src] + wf * rsh(zs);
+        if (!direct && z == 0)
+            h1o[(d - 1) * S + s] = -((double)h[(opp(d) - 1) * S + s] + wf * qsh(0)) + P.inj[WCLS[d]];
+        else
+            h1o[(d - 1) * S + s] = (double)h[(d - 1) * S + src] + wf * qsh(zs);
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) write_pre_kernel(LbmParams P, int shifted, const double* f1,
-                                                        const double* f0, const double* h1,
-                                                        const double* h0, T* f, T* f0o, T* h, T* h0o)
+__global__ void __launch_bounds__(256) write_pre_kernel(LbmParams P, const double* f1, const double* f0,
+                                                        const double* h1, const double* h0, T* f, T* f0o, T* h,
+                                                        T* h0o)
 {
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= P.S) return;
     const unsigned S = P.S;
-    const double rs = shifted ? P.rho0 : 0.0, qs = shifted ? P.q_ref : 0.0;
+    const unsigned z = s / P.nxy;
+    const double rs = P.rs ? P.rs[z] : 0.0, qs = P.qs ? P.qs[z] : 0.0;
     f0o[s] = (T)(f0[s] - P.w[0] * rs);
     h0o[s] = (T)(h0[s] - P.w[0] * qs);
 #pragma unroll
@@ -450,10 +452,10 @@ hipError_t launch_init_equilibrium(Prec prec, const LbmParams& P, const double* 
     dim3 g = grid_for(P.S), b(256);
     if (prec == Prec::F64)
         init_eq_kernel<double><<<g, b, 0, st>>>(P, rho, q, ux, uy, uz, ex, ey, ez, (double*)f, (double*)f0,
-                                                (double*)h, (double*)h0, 0);
+                                                (double*)h, (double*)h0);
     else
         init_eq_kernel<float><<<g, b, 0, st>>>(P, rho, q, ux, uy, uz, ex, ey, ez, (float*)f, (float*)f0,
-                                               (float*)h, (float*)h0, 1);
+                                               (float*)h, (float*)h0);
     return hipGetLastError();
 }
 
@@ -463,10 +465,10 @@ hipError_t launch_read_pre(Prec prec, const LbmParams& P, bool direct, const voi
 {
     dim3 g = grid_for(P.S), b(256);
     if (prec == Prec::F64)
-        read_pre_kernel<double><<<g, b, 0, st>>>(P, direct ? 1 : 0, 0, (const double*)f, (const double*)f0,
+        read_pre_kernel<double><<<g, b, 0, st>>>(P, direct ? 1 : 0, (const double*)f, (const double*)f0,
                                                  (const double*)h, (const double*)h0, f1o, f0o, h1o, h0o);
     else
-        read_pre_kernel<float><<<g, b, 0, st>>>(P, direct ? 1 : 0, 1, (const float*)f, (const float*)f0,
+        read_pre_kernel<float><<<g, b, 0, st>>>(P, direct ? 1 : 0, (const float*)f, (const float*)f0,
                                                 (const float*)h, (const float*)h0, f1o, f0o, h1o, h0o);
     return hipGetLastError();
 }
@@ -477,10 +479,10 @@ hipError_t launch_write_pre(Prec prec, const LbmParams& P, const double* f1, con
 {
     dim3 g = grid_for(P.S), b(256);
     if (prec == Prec::F64)
-        write_pre_kernel<double><<<g, b, 0, st>>>(P, 0, f1, f0, h1, h0, (double*)f, (double*)f0o, (double*)h,
+        write_pre_kernel<double><<<g, b, 0, st>>>(P, f1, f0, h1, h0, (double*)f, (double*)f0o, (double*)h,
                                                   (double*)h0o);
     else
-        write_pre_kernel<float><<<g, b, 0, st>>>(P, 1, f1, f0, h1, h0, (float*)f, (float*)f0o, (float*)h,
+        write_pre_kernel<float><<<g, b, 0, st>>>(P, f1, f0, h1, h0, (float*)f, (float*)f0o, (float*)h,
                                                  (float*)h0o);
     return hipGetLastError();
 }

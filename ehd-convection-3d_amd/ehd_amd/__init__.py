@@ -38,7 +38,7 @@ class Params(C.Structure):
         ("cfl", C.c_double), ("voltage", C.c_double), ("eps", C.c_double), ("rho0", C.c_double),
         ("lz", C.c_double), ("uw", C.c_double), ("exf", C.c_double),
         ("precision", C.c_int), ("init_mode", C.c_int),
-        ("seed", C.c_uint64), ("perturb_amp", C.c_double),
+        ("seed", C.c_uint64), ("perturb_amp", C.c_double), ("hydro_density", C.c_int),
     ]
 
 
@@ -133,13 +133,15 @@ def default_params() -> Params:
 
 def make_params(nx, ny, nz, T=200.0, C_=10.0, M=10.0, Fe=4000.0, Pr=10.0, lam=0.25, lam_c=1e-6,
                 precision=FP64, init_mode=INIT_HYDROSTATIC, seed=1, amp=1e-3,
-                cfl=0.01, voltage=1.0e4, eps=1.0e-4, rho0=1600.0, lz=1.0, uw=0.0, exf=0.0) -> Params:
+                cfl=0.01, voltage=1.0e4, eps=1.0e-4, rho0=1600.0, lz=1.0, uw=0.0, exf=0.0,
+                hydro_density=0) -> Params:
     p = default_params()
     p.nx, p.ny, p.nz = nx, ny, nz
     p.T, p.C, p.M, p.Fe, p.Pr = T, C_, M, Fe, Pr
     p.lambda_f, p.lambda_c = lam, lam_c
     p.cfl, p.voltage, p.eps, p.rho0, p.lz, p.uw, p.exf = cfl, voltage, eps, rho0, lz, uw, exf
     p.precision, p.init_mode, p.seed, p.perturb_amp = precision, init_mode, seed, amp
+    p.hydro_density = hydro_density
     return p
 
 

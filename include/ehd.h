@@ -34,7 +34,7 @@ extern "C" {
 
 /* storage / arithmetic of the populations */
 #define EHD_FP64           0   /* double storage, double arithmetic, reference operation order */
-#define EHD_FP32S          1   /* float storage of f - w*rho0 and h - w*q_ref, float arithmetic */
+#define EHD_FP32S          1   /* float storage of f - w*rs(z), h - w*qs(z) (hydrostatic shifts), float arithmetic */
 
 /* initial state */
 #define EHD_INIT_REFERENCE   0 /* rho0, q=0, linear phi, u=0, E=0 (gpu_initialization, LBM.cu:80-96) */
@@ -57,6 +57,8 @@ typedef struct ehd_params {
     int    init_mode;         /* EHD_INIT_* */
     uint64_t seed;            /* perturbation seed (EHD_INIT_HYDROSTATIC) */
     double perturb_amp;       /* perturbation amplitude of u, physical units (1e-3) */
+    int    hydro_density;     /* EHD_INIT_HYDROSTATIC: 0 = uniform rho0 (as gpu_initialization),
+                                 1 = hydrostatic pressure, rho0 + (p - mean p)/cs^2 with dp/dz = q Ez */
 } ehd_params;
 
 /* Dimensional constants derived from ehd_params (inverse of compute_parameters). */

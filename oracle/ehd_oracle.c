@@ -235,30 +235,68 @@ static void fft_rec(const fftplan* P, const double* in, double* out, int n, int 
     int m = n / p;
     for (int r = 0; r < p; ++r)
         fft_rec(P, in + 2 * (size_t)r * istride, out + 2 * (size_t)r * m, m, istride * p, fi + 1, sign);
-    int tstep = P->n / n;
-    double t[2 * 64];
-    double* tt = (p <= 64) ? t : (double*)malloc(sizeof(double) * 2 * (size_t)p);
+    const int tstep = P->n / n;          /* W_n^j = tw[j*tstep]; r*k*tstep < P->n */
+    const double* tw = P->tw;
+    const double sg = (double)sign;
+#define TWR(j) (tw[2 * (size_t)(j)])
+#define TWI(j) (sg * tw[2 * (size_t)(j) + 1])
+    if (p == 2) {
+        for (int k = 0; k < m; ++k) {
+            double* A = out + 2 * (size_t)k;
+            double* B = out + 2 * ((size_t)m + k);
+            int j = k * tstep;
+            double br = B[0] * TWR(j) - B[1] * TWI(j), bi = B[0] * TWI(j) + B[1] * TWR(j);
+            double ar = A[0], ai = A[1];
+            A[0] = ar + br; A[1] = ai + bi;
+            B[0] = ar - br; B[1] = ai - bi;
+        }
+        return;
+    }
+    if (p == 4) {
+        for (int k = 0; k < m; ++k) {
+            double* X0 = out + 2 * (size_t)k;
+            double* X1 = out + 2 * ((size_t)m + k);
+            double* X2 = out + 2 * (2 * (size_t)m + k);
+            double* X3 = out + 2 * (3 * (size_t)m + k);
+            int j1 = k * tstep, j2 = 2 * k * tstep, j3 = 3 * k * tstep;
+            double ar = X0[0], ai = X0[1];
+            double br = X1[0] * TWR(j1) - X1[1] * TWI(j1), bi = X1[0] * TWI(j1) + X1[1] * TWR(j1);
+            double cr = X2[0] * TWR(j2) - X2[1] * TWI(j2), ci = X2[0] * TWI(j2) + X2[1] * TWR(j2);
+            double dr = X3[0] * TWR(j3) - X3[1] * TWI(j3), di = X3[0] * TWI(j3) + X3[1] * TWR(j3);
+            double t0r = ar + cr, t0i = ai + ci, t1r = ar - cr, t1i = ai - ci;
+            double t2r = br + dr, t2i = bi + di, t3r = br - dr, t3i = bi - di;
+            double ir = -sg * t3i, ii = sg * t3r;     /* W4 = i*sign */
+            X0[0] = t0r + t2r; X0[1] = t0i + t2i;
+            X1[0] = t1r + ir;  X1[1] = t1i + ii;
+            X2[0] = t0r - t2r; X2[1] = t0i - t2i;
+            X3[0] = t1r - ir;  X3[1] = t1i - ii;
+        }
+        return;
+    }
+    double t[2 * 64], wp[2 * 64];
+    for (int j = 0; j < p; ++j) { wp[2 * j] = TWR((size_t)j * (P->n / p)); wp[2 * j + 1] = TWI((size_t)j * (P->n / p)); }
     for (int k = 0; k < m; ++k) {
         for (int r = 0; r < p; ++r) {
             double xr = out[2 * ((size_t)r * m + k)], xi = out[2 * ((size_t)r * m + k) + 1];
-            int j = (int)(((long long)r * k * tstep) % P->n);
-            double c = P->tw[2 * j], s = sign * P->tw[2 * j + 1];
-            tt[2 * r] = xr * c - xi * s;
-            tt[2 * r + 1] = xr * s + xi * c;
+            int j = r * k * tstep;
+            t[2 * r] = xr * TWR(j) - xi * TWI(j);
+            t[2 * r + 1] = xr * TWI(j) + xi * TWR(j);
         }
         for (int q = 0; q < p; ++q) {
             double ar = 0.0, ai = 0.0;
+            int idx = 0;
             for (int r = 0; r < p; ++r) {
-                int j = (int)(((long long)r * q * m * tstep) % P->n);
-                double c = P->tw[2 * j], s = sign * P->tw[2 * j + 1];
-                ar += tt[2 * r] * c - tt[2 * r + 1] * s;
-                ai += tt[2 * r] * s + tt[2 * r + 1] * c;
+                ar += t[2 * r] * wp[2 * idx] - t[2 * r + 1] * wp[2 * idx + 1];
+                ai += t[2 * r] * wp[2 * idx + 1] + t[2 * r + 1] * wp[2 * idx];
+                idx += q;
+                if (idx >= p) idx -= p;
             }
             out[2 * ((size_t)q * m + k)] = ar;
             out[2 * ((size_t)q * m + k) + 1] = ai;
         }
     }
-    if (tt != t) free(tt);
+#undef TWR
+#undef TWI
 }
 
 /* data: 2n doubles, transformed in place.  work: at least 2*max(n, 2m) doubles. */
@@ -470,24 +508,28 @@ double oref_hydrostatic_e0(double C)
     return 0.5 * (lo + hi);
 }
 
-void oref_init_hydrostatic(oref_box* b, uint64_t seed, double amp)
+void oref_init_hydrostatic(oref_box* b, uint64_t seed, double amp, int hydro_density)
 {
     const oref_params* p = &b->p;
     unsigned NX = p->nx, NY = p->ny, NZ = p->nz;
     double C = p->charge0 * p->lz * p->lz / (p->voltage * p->eps);
     double e0 = oref_hydrostatic_e0(C);
     double qs = p->eps * p->voltage / (p->lz * p->lz);
+    /* uniform current density J = q Ez balances the pressure: dp/dz = J,
+     * rho = rho0 + J (z - Lz/2)/cs^2 (mean rho0) */
+    double J = C * e0 * p->eps * p->voltage * p->voltage / (p->lz * p->lz * p->lz);
     for (unsigned z = 0; z < NZ; ++z) {
         double zeta = p->dz * z / p->lz;
         double e = sqrt(e0 * e0 + 2.0 * C * e0 * zeta);
         double qz = C * e0 / e * qs;
         double phz = p->voltage * (1.0 - (e * e * e - e0 * e0 * e0) / (3.0 * C * e0));
+        double rhoz = hydro_density ? p->rho0 + J * (p->dz * z - 0.5 * p->lz) / p->cs_square : p->rho0;
         if (z == 0) phz = p->voltage;
         if (z == NZ - 1) phz = 0.0;
         for (unsigned y = 0; y < NY; ++y)
             for (unsigned x = 0; x < NX; ++x) {
                 size_t s = SIDX(x, y, z);
-                b->fld[OREF_F_RHO][s] = p->rho0;
+                b->fld[OREF_F_RHO][s] = rhoz;
                 b->fld[OREF_F_Q][s] = qz;
                 b->fld[OREF_F_PHI][s] = phz;
                 int wall = (z == 0 || z == NZ - 1);

@@ -69,7 +69,10 @@ double* oref_field(oref_box* b, int which);
 /* Initial states. */
 void oref_initialization(oref_box* b);                 /* LBM.cu:80-96 */
 void oref_set_fields(oref_box* b, const double* f9);   /* read_data semantics (LBM.cu:1596-1643): rho,q,phi,u,v,w,Ex,Ey,Ez */
-void oref_init_hydrostatic(oref_box* b, uint64_t seed, double amp);
+/* Hydrostatic charge and potential plus a seeded perturbation of u in the
+ * interior; hydro_density = 1 also sets the hydrostatic pressure (rho0 +
+ * (p - mean p)/cs^2, dp/dz = q Ez), 0 keeps rho = rho0 (gpu_initialization). */
+void oref_init_hydrostatic(oref_box* b, uint64_t seed, double amp, int hydro_density);
 void oref_init_equilibrium(oref_box* b);               /* LBM.cu:98-276 */
 
 /* LBM phases (LBM.cu:278-1353). */

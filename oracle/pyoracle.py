@@ -69,7 +69,7 @@ def lib() -> C.CDLL:
             getattr(L, name).argtypes = [vp]
             getattr(L, name).restype = None
         L.oref_set_fields.argtypes = [vp, dp]
-        L.oref_init_hydrostatic.argtypes = [vp, C.c_uint64, C.c_double]
+        L.oref_init_hydrostatic.argtypes = [vp, C.c_uint64, C.c_double, C.c_int]
         L.oref_fft3d.argtypes = [vp, dp, dp, C.c_int]
         L.oref_step.argtypes = [vp, C.c_int]
         L.oref_current.argtypes = [vp]
@@ -175,7 +175,8 @@ class Box:
     # phases
     def initialization(self): lib().oref_initialization(self._b)
     def init_equilibrium(self): lib().oref_init_equilibrium(self._b)
-    def init_hydrostatic(self, seed: int, amp: float): lib().oref_init_hydrostatic(self._b, seed, amp)
+    def init_hydrostatic(self, seed: int, amp: float, hydro_density: int = 0):
+        lib().oref_init_hydrostatic(self._b, seed, amp, int(hydro_density))
 
     def set_fields(self, f9: np.ndarray):
         a = np.ascontiguousarray(f9, dtype=np.float64)

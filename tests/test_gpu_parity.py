@@ -20,11 +20,12 @@ pytestmark = pytest.mark.gpu
 FIELDS = (O.RHO, O.Q, O.PHI, O.UX, O.UY, O.UZ, O.EX, O.EY, O.EZ)
 
 
-def oracle_box(nx, ny, nz, T=200.0, C=10.0, lam=0.25, uw=0.0, exf=0.0, seed=1, amp=1e-3, mode="hydro"):
+def oracle_box(nx, ny, nz, T=200.0, C=10.0, lam=0.25, uw=0.0, exf=0.0, seed=1, amp=1e-3, mode="hydro",
+               hydro_density=0):
     ob = O.Box(O.make_params(nx, ny, nz, T=T, M=10.0, C_=C, Fe=4000.0, lam=lam, uw=uw, exf=exf))
     ob.groups = dict(T=T, C_=C, M=10.0, Fe=4000.0)
     if mode == "hydro":
-        ob.init_hydrostatic(seed, amp)
+        ob.init_hydrostatic(seed, amp, hydro_density)
     else:
         ob.initialization()
     return ob
@@ -79,12 +80,13 @@ def test_init_equilibrium_bitexact(shape):
         compare_pops(b, ob, "init")
 
 
-def test_product_hydrostatic_init_matches_oracle():
+@pytest.mark.parametrize("hd", [0, 1])
+def test_product_hydrostatic_init_matches_oracle(hd):
     nx, ny, nz = 16, 12, 10
-    ob = oracle_box(nx, ny, nz, seed=5, amp=1e-3)
+    ob = oracle_box(nx, ny, nz, seed=5, amp=1e-3, hydro_density=hd)
     ob.init_equilibrium()
     p = E.make_params(nx, ny, nz, T=200.0, C_=10.0, lam=0.25, precision=E.FP64,
-                      init_mode=E.INIT_HYDROSTATIC, seed=5, amp=1e-3)
+                      init_mode=E.INIT_HYDROSTATIC, seed=5, amp=1e-3, hydro_density=hd)
     with E.Box(p) as b:
         b.init()
         f = b.read_fields()
@@ -167,7 +169,8 @@ def test_random_population_state_bitexact():
             compare_pops(b, ob, f"random step {k + 1}")
 
 
-@pytest.mark.parametrize("shape", [(8, 6, 7), (48, 48, 32), (80, 80, 12), (64, 64, 32), (122, 122, 10), (96, 96, 8)])
+@pytest.mark.parametrize("shape", [(8, 6, 7), (48, 48, 32), (80, 80, 12), (64, 64, 32), (122, 122, 10), (96, 96, 8),
+                                   (64, 48, 200), (128, 128, 65), (16, 8, 300)])
 @pytest.mark.parametrize("precision", [E.FP64, E.FP32S])
 def test_poisson_vs_oracle(shape, precision):
     nx, ny, nz = shape
@@ -203,7 +206,7 @@ def test_diagnostics_current_vs_oracle():
         assert d.step == 3
 
 
-@pytest.mark.parametrize("nsteps", [200])
+@pytest.mark.parametrize("nsteps", [1000])
 def test_coupled_fp64_config1_vs_oracle(nsteps):
     """64x64x32, T=200, C=10, M=10, Fe=4000, Lambda=1/4, hydrostatic start + 1e-3 noise."""
     ob = oracle_box(64, 64, 32)
@@ -223,9 +226,10 @@ def test_coupled_fp64_config1_vs_oracle(nsteps):
         assert v < 1e-12, (k, v)
 
 
-def test_fp32s_vs_fp64_short():
+@pytest.mark.parametrize("hd", [0, 1])
+def test_fp32s_vs_fp64_short(hd):
     """FP32 storage of shifted populations against the FP64 path, same start."""
-    ob = oracle_box(32, 32, 16)
+    ob = oracle_box(32, 32, 16, hydro_density=hd)
     ob.init_equilibrium()
     res = {}
     for prec in (E.FP64, E.FP32S):
@@ -234,6 +238,6 @@ def test_fp32s_vs_fp64_short():
             res[prec] = b.read_fields(("q", "phi", "ux", "uy", "uz"))
     a, r = res[E.FP32S], res[E.FP64]
     errs = {k: rel_l2(a[k][1:-1], r[k][1:-1]) for k in ("q", "phi", "ux", "uy", "uz")}
-    print(f"FP32S vs FP64 after 300 steps: {errs}")
+    print(f"FP32S vs FP64 after 300 steps (hydro_density={hd}): {errs}")
     assert errs["q"] < 1e-5 and errs["phi"] < 1e-5
     assert max(errs["ux"], errs["uy"], errs["uz"]) < 1e-2

@@ -414,15 +414,22 @@ def test_hash_perturbation():
 
 def test_init_hydrostatic_state():
     b = _box(6, 4, 11, C_=10.0)
-    b.init_hydrostatic(3, 1e-3)
+    b.init_hydrostatic(3, 1e-3, hydro_density=1)
     p = b.p
     assert np.all(b[O.PHI][0] == p.voltage) and np.all(b[O.PHI][-1] == 0.0)
     assert np.allclose(b[O.Q][0], p.charge0, rtol=1e-14)
     assert np.all(b[O.UX][0] == 0) and np.all(b[O.UZ][-1] == 0)
     assert 0 < np.max(np.abs(b[O.UX][1:-1])) <= 1e-3
+    # hydrostatic pressure: dp/dz = q Ez (uniform), mean density rho0
+    assert abs(b[O.RHO].mean() - p.rho0) < 1e-9 * p.rho0
+    dp = np.diff(b[O.RHO][:, 0, 0]) * p.cs_square / p.dz
+    qE = p.charge0 * p.voltage / p.lz * O.hydrostatic_e0(10.0)   # q(0) E(0)
+    assert np.allclose(dp, qE, rtol=1e-9)
     b.init_equilibrium()
     f = np.concatenate([b[O.F0][None], b[O.F1]])
-    assert np.allclose(f.sum(0), p.rho0, rtol=1e-14)
+    assert np.allclose(f.sum(0), b[O.RHO], rtol=1e-14)
+    b.init_hydrostatic(3, 1e-3, hydro_density=0)
+    assert np.all(b[O.RHO] == p.rho0)
 
 
 def test_coupled_step_physics_sanity():
@@ -435,3 +442,26 @@ def test_coupled_step_physics_sanity():
     assert np.max(np.abs(b[O.UX][1:-1])) < 1e-9
     assert np.max(np.abs(np.diff(b[O.Q][1:-1], axis=2))) < 1e-9
     assert np.min(b[O.Q][1:-1]) > 0 and b.current() > 0
+
+
+def test_force_population_structure_matches_reference_text():
+    """Term order and signs of every Guo force population (LBM.cu:671-702), as parsed
+    from the source text, follow the rule the oracle evaluates: for each nonzero axis a
+    of c_d a term (c_a/CFL - u_a + P_a cflinv2) F_a with P_a = sum_b c_a c_b u_b over the
+    nonzero axes in x,y,z order, and -u_b F_b for the zero axes; one-axis directions
+    put the -u_b F_b terms first, the others put them last."""
+    c = O.tables()["c"]
+    for d in range(27):
+        ref = TABLES["fpop"][str(d)]
+        cls = 0 if d == 0 else (1 if d <= 6 else (2 if d <= 18 else 3))
+        assert ref["coe"] == cls and ref["neg"] == (d == 0)
+        if d == 0:
+            assert [(t["kind"], t["sign"], t["axis"]) for t in ref["terms"]] == [("U", 1, 1), ("U", 1, 2), ("U", 1, 3)]
+            continue
+        nz_axes = [a for a in range(3) if c[d][a] != 0]
+        zero_axes = [a for a in range(3) if c[d][a] == 0]
+        a_terms = [{"kind": "A", "sign": 1, "axis": a + 1, "head": int(c[d][a]),
+                    "P": [int(c[d][a] * c[d][b]) * (b + 1) for b in nz_axes]} for a in nz_axes]
+        u_terms = [{"kind": "U", "sign": -1, "axis": a + 1} for a in zero_axes]
+        expect = (u_terms + a_terms) if len(nz_axes) == 1 else (a_terms + u_terms)
+        assert ref["terms"] == expect, d
