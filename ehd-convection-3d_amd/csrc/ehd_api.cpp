@@ -130,6 +130,8 @@ int validate(const ehd_params* p)
     if (!(p->lambda_f > 0) || !(p->lambda_c > 0)) return fail(EHD_EINVAL, "TRT magic parameters must be positive");
     if (p->precision != EHD_FP64 && p->precision != EHD_FP32S) return fail(EHD_EINVAL, "unknown precision");
     if (p->init_mode < EHD_INIT_REFERENCE || p->init_mode > EHD_INIT_FIELDS) return fail(EHD_EINVAL, "unknown init mode");
+    if (p->perturb_pattern < EHD_PERTURB_NOISE || p->perturb_pattern > EHD_PERTURB_HEXAGONS)
+        return fail(EHD_EINVAL, "unknown perturbation pattern");
     return EHD_OK;
 }
 
@@ -453,8 +455,8 @@ int ehd_init(ehd_box* b, const double* fields9)
     } guard{tmp};
     double* dqz = nullptr;
     if (p.init_mode == EHD_INIT_REFERENCE) {
-        HIPCHK(launch_init_fields(b->lp, 0, nullptr, nullptr, nullptr, 0, 0.0, p.voltage, p.lz, rho, q, phi, ux, uy, uz,
-                                  b->st));
+        HIPCHK(launch_init_fields(b->lp, 0, nullptr, nullptr, nullptr, 0, 0.0, 0, b->der.lx, b->der.ly, p.voltage, p.lz,
+                                  rho, q, phi, ux, uy, uz, b->st));
         HIPCHK(hipMemsetAsync(b->ex, 0, S * sizeof(double), b->st));
         HIPCHK(hipMemsetAsync(b->ey, 0, S * sizeof(double), b->st));
         HIPCHK(hipMemsetAsync(b->ez, 0, S * sizeof(double), b->st));
@@ -479,8 +481,9 @@ int ehd_init(ehd_box* b, const double* fields9)
         rc = alloc((void**)&dqz, tab.size() * sizeof(double));
         if (rc) return rc;
         hipError_t e1 = hipMemcpyAsync(dqz, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, b->st);
-        hipError_t e2 = launch_init_fields(b->lp, 1, dqz, dqz + nz, dqz + 2 * nz, p.seed, p.perturb_amp, p.voltage,
-                                           p.lz, rho, q, phi, ux, uy, uz, b->st);
+        hipError_t e2 = launch_init_fields(b->lp, 1, dqz, dqz + nz, dqz + 2 * nz, p.seed, p.perturb_amp,
+                                           p.perturb_pattern, b->der.lx, b->der.ly, p.voltage, p.lz, rho, q, phi, ux,
+                                           uy, uz, b->st);
         hipError_t e3 = launch_efield(Prec::F64, b->lp, phi, nullptr, b->ex, b->ey, b->ez, b->st);
         hipError_t e4 = hipStreamSynchronize(b->st);
         (void)hipFree(dqz);
@@ -680,6 +683,94 @@ int ehd_write_populations(ehd_box* b, const double* f0, const double* f1, const 
     HIPCHK(hipMemcpyAsync(b->f0[B], b->f0[A], S * b->esz, hipMemcpyDeviceToDevice, b->st));
     HIPCHK(hipStreamSynchronize(b->st));
     b->direct = true;
+    return EHD_OK;
+}
+
+namespace {
+struct CkptHeader {
+    char magic[8];
+    int abi, precision, nx, ny, nz, direct, ebuf, pad;
+    long long step;
+    double t;
+    double T, C, M, Fe, lambda_f, lambda_c;
+};
+
+int ckpt_io(ehd_box* b, FILE* f, bool save)
+{
+    const size_t S = b->S, es = b->esz;
+    const int A = b->cur, B = 1 - b->cur;
+    struct Item {
+        void* dev;
+        size_t bytes;
+    } items[] = {
+        {b->f[A], 26 * S * es}, {b->f0[A], S * es}, {b->h[A], 26 * S * es}, {b->h0[A], S * es},
+        {b->f0[B], S * es},    // frozen wall rest populations live in both buffers
+        {b->rho, S * es}, {b->ux, S * es}, {b->uy, S * es}, {b->uz, S * es}, {b->q, S * es}, {b->phi, S * es},
+        {b->phibar, (size_t)b->nz * sizeof(double)},
+        {b->ex, S * sizeof(double)}, {b->ey, S * sizeof(double)}, {b->ez, S * sizeof(double)},
+    };
+    std::vector<unsigned char> buf;
+    for (auto& it : items) {
+        buf.resize(it.bytes);
+        if (save) {
+            HIPCHK(hipMemcpy(buf.data(), it.dev, it.bytes, hipMemcpyDeviceToHost));
+            if (std::fwrite(buf.data(), 1, it.bytes, f) != it.bytes) return fail(EHD_EINVAL, "checkpoint write failed");
+        } else {
+            if (std::fread(buf.data(), 1, it.bytes, f) != it.bytes) return fail(EHD_EINVAL, "checkpoint truncated");
+            HIPCHK(hipMemcpy(it.dev, buf.data(), it.bytes, hipMemcpyHostToDevice));
+        }
+    }
+    return EHD_OK;
+}
+}  // namespace
+
+int ehd_save_state(ehd_box* b, const char* path)
+{
+    if (!b || !path) return fail(EHD_EINVAL, "null argument");
+    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    HIPCHK(hipStreamSynchronize(b->st));
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(EHD_EINVAL, "cannot open %s for writing", path);
+    CkptHeader h{};
+    std::memcpy(h.magic, "EHDCKPT1", 8);
+    h.abi = EHD_ABI_VERSION;
+    h.precision = b->prm.precision;
+    h.nx = b->nx; h.ny = b->ny; h.nz = b->nz;
+    h.direct = b->direct ? 1 : 0;
+    h.ebuf = b->ebuf ? 1 : 0;
+    h.step = b->step;
+    h.t = b->t;
+    h.T = b->prm.T; h.C = b->prm.C; h.M = b->prm.M; h.Fe = b->prm.Fe;
+    h.lambda_f = b->prm.lambda_f; h.lambda_c = b->prm.lambda_c;
+    int rc = (std::fwrite(&h, sizeof(h), 1, f) == 1) ? ckpt_io(b, f, true) : fail(EHD_EINVAL, "checkpoint write failed");
+    if (std::fclose(f) != 0 && rc == EHD_OK) rc = fail(EHD_EINVAL, "checkpoint close failed");
+    return rc;
+}
+
+int ehd_load_state(ehd_box* b, const char* path)
+{
+    if (!b || !path) return fail(EHD_EINVAL, "null argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(EHD_EINVAL, "cannot open %s", path);
+    CkptHeader h{};
+    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, "EHDCKPT1", 8) != 0) {
+        std::fclose(f);
+        return fail(EHD_EINVAL, "%s is not a checkpoint", path);
+    }
+    if (h.precision != b->prm.precision || h.nx != b->nx || h.ny != b->ny || h.nz != b->nz) {
+        std::fclose(f);
+        return fail(EHD_EINVAL, "checkpoint grid/precision (%dx%dx%d, %d) differ from the box", h.nx, h.ny, h.nz,
+                    h.precision);
+    }
+    HIPCHK(hipStreamSynchronize(b->st));
+    const int rc = ckpt_io(b, f, false);
+    std::fclose(f);
+    if (rc) return rc;
+    b->direct = h.direct != 0;
+    b->ebuf = h.ebuf != 0;
+    b->step = h.step;
+    b->t = h.t;
+    b->initialised = true;
     return EHD_OK;
 }
 

@@ -87,9 +87,9 @@ hipError_t launch_phi_split(Prec prec, const LbmParams& P, const double* phi, vo
 
 // Reference and hydrostatic initial fields (doubles).
 hipError_t launch_init_fields(const LbmParams& P, int mode, const double* qz, const double* phiz,
-                              const double* rhoz, uint64_t seed, double amp, double voltage, double lz, double* rho,
-                              double* q, double* phi, double* ux, double* uy, double* uz,
-                              hipStream_t st);
+                              const double* rhoz, uint64_t seed, double amp, int pattern, double lx, double ly,
+                              double voltage, double lz, double* rho, double* q, double* phi, double* ux,
+                              double* uy, double* uz, hipStream_t st);
 int diag_blocks();
 
 // --- Poisson (poisson.hip) -------------------------------------------------

@@ -28,11 +28,42 @@ __device__ __forceinline__ double hash_uniform(uint64_t seed, uint32_t x, uint32
     return 2.0 * u - 1.0;
 }
 
+// Velocity perturbation patterns (EHD_PERTURB_*), after the roll, square,
+// crossed-roll and hexagon fields of LBM.cu:389-411 with amplitude amp and
+// coordinates zeta = z/Lz, xi = x/Lx, eta = y/Ly.
+__device__ void pattern_velocity(int pattern, double amp, double zeta, double xi, double eta, double xx, double yy,
+                                 double lx, double ly, double lz, double* u)
+{
+    const double tp = 2.0 * 3.14159265358979323846;
+    const double cz = cos(tp * zeta) - 1.0, sz = sin(tp * zeta);
+    u[0] = u[1] = u[2] = 0.0;
+    if (pattern == EHD_PERTURB_ROLLS) {
+        u[2] = amp * cz * cos(tp * xi);
+        u[0] = amp * (lx / lz) * sz * sin(tp * xi);
+    } else if (pattern == EHD_PERTURB_SQUARES) {
+        u[2] = amp * cz * cos(tp * xi) * cos(tp * eta);
+        u[0] = 0.5 * amp * (lx / lz) * sz * sin(tp * xi) * cos(tp * eta);
+        u[1] = 0.5 * amp * (ly / lz) * sz * sin(tp * eta) * cos(tp * xi);
+    } else if (pattern == EHD_PERTURB_CROSS) {
+        u[2] = amp * cz * (cos(tp * xi) + cos(tp * eta));
+        u[0] = amp * (lx / lz) * sz * sin(tp * xi);
+        u[1] = amp * (ly / lz) * sz * sin(tp * eta);
+    } else if (pattern == EHD_PERTURB_HEXAGONS) {
+        const double L = lx / sqrt(3.0);
+        const double a = 4.0 * 3.14159265358979323846 / 3.0 / L;
+        const double cx = cos(tp / (sqrt(3.0) * L) * xx), sx = sin(tp / (sqrt(3.0) * L) * xx);
+        const double cy = cos(tp / (3.0 * L) * yy), sy = sin(tp / (3.0 * L) * yy);
+        u[2] = -amp * cz / 3.0 * (2.0 * cx * cy + cos(2.0 * tp / (3.0 * L) * yy));
+        u[0] = -amp * tp * sz * (4.0 * 3.14159265358979323846) / (3.0 * sqrt(3.0) * L * a * a) * sx * cy;
+        u[1] = -amp * tp * sz * (4.0 * 3.14159265358979323846) / (9.0 * L * a * a) * (cx + 2.0 * cy) * sy;
+    }
+}
+
 // mode 0: gpu_initialization (LBM.cu:80-96); mode 1: hydrostatic profile
-// (tables qz, phiz per plane) plus the seeded perturbation of u.
+// (tables qz, phiz, rhoz per plane) plus the perturbation of u.
 __global__ void __launch_bounds__(256) init_fields_kernel(LbmParams P, int mode, const double* qz,
                                                           const double* phiz, const double* rhoz, uint64_t seed,
-                                                          double amp,
+                                                          double amp, int pattern, double lx, double ly,
                                                           double voltage, double lz, double* rho, double* q,
                                                           double* phi, double* ux, double* uy, double* uz)
 {
@@ -51,9 +82,18 @@ __global__ void __launch_bounds__(256) init_fields_kernel(LbmParams P, int mode,
         q[s] = qz[z];
         phi[s] = phiz[z];
         const bool wall = (z == 0) || (z == (unsigned)P.nz - 1);
-        ux[s] = wall ? 0.0 : amp * hash_uniform(seed, x, y, z, 0);
-        uy[s] = wall ? 0.0 : amp * hash_uniform(seed, x, y, z, 1);
-        uz[s] = wall ? 0.0 : amp * hash_uniform(seed, x, y, z, 2);
+        if (pattern == EHD_PERTURB_NOISE) {
+            ux[s] = wall ? 0.0 : amp * hash_uniform(seed, x, y, z, 0);
+            uy[s] = wall ? 0.0 : amp * hash_uniform(seed, x, y, z, 1);
+            uz[s] = wall ? 0.0 : amp * hash_uniform(seed, x, y, z, 2);
+        } else {
+            double u[3];
+            pattern_velocity(pattern, amp, P.dz * z / lz, P.dx * x / lx, P.dy * y / ly, P.dx * x, P.dy * y, lx, ly,
+                             lz, u);
+            ux[s] = wall ? 0.0 : u[0];
+            uy[s] = wall ? 0.0 : u[1];
+            uz[s] = wall ? 0.0 : u[2];
+        }
     }
 }
 
@@ -208,11 +248,11 @@ __global__ void __launch_bounds__(256) diag_final_kernel(const double* part, int
 }  // namespace
 
 hipError_t launch_init_fields(const LbmParams& P, int mode, const double* qz, const double* phiz, const double* rhoz,
-                              uint64_t seed, double amp, double voltage, double lz, double* rho, double* q, double* phi,
-                              double* ux, double* uy, double* uz, hipStream_t st)
+                              uint64_t seed, double amp, int pattern, double lx, double ly, double voltage, double lz,
+                              double* rho, double* q, double* phi, double* ux, double* uy, double* uz, hipStream_t st)
 {
-    init_fields_kernel<<<grid_for(P.S), 256, 0, st>>>(P, mode, qz, phiz, rhoz, seed, amp, voltage, lz, rho, q, phi,
-                                                      ux, uy, uz);
+    init_fields_kernel<<<grid_for(P.S), 256, 0, st>>>(P, mode, qz, phiz, rhoz, seed, amp, pattern, lx, ly, voltage, lz,
+                                                      rho, q, phi, ux, uy, uz);
     return hipGetLastError();
 }
 

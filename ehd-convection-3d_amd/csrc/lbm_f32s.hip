@@ -52,16 +52,19 @@ __device__ __forceinline__ float pulled(float v, int d, float w, float dup, floa
     return v;
 }
 
+// Every population is read once and written once per step: streaming
+// (non-temporal) loads and stores keep them from displacing the potential
+// and the shift tables in the caches.
+__device__ __forceinline__ float ldnt(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stnt(float v, float* p) { __builtin_nontemporal_store(v, p); }
+
+// Blocks run in linear order (x fastest, then y, then z): measured faster on
+// MI355X than a remap that gives each XCD a contiguous slab of planes.
 template <bool FIRST, bool EBUF>
 __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c, PopBufs pb, MacroBufs mb,
-                                                       int macro, unsigned nblk_xcd)
+                                                       int macro)
 {
-    // XCD-aware block order: hardware block i runs on XCD i%8; give each XCD a
-    // contiguous range of rows so the phi stencil of neighbouring rows stays
-    // in that XCD's L2.
-    unsigned b = blockIdx.x;
-    if (nblk_xcd) b = (b & 7u) * nblk_xcd + (b >> 3);
-    const unsigned s = b * blockDim.x + threadIdx.x;
+    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= P.S) return;
     const unsigned nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy, S = P.S;
     const unsigned x = s % nx, t = s / nx, y = t % ny, z = t / ny;
@@ -93,7 +96,7 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
     } else {
 #pragma unroll
         for (int d = 1; d < 27; ++d)
-            g[d] = pulled(fA[(d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]], d, c.w[WCLS[d]], dRu, dRd);
+            g[d] = pulled(ldnt(fA + (d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]), d, c.w[WCLS[d]], dRu, dRd);
         if (z == 0) {
             k[0] = -h0A[s] + c.injk[0];
 #pragma unroll
@@ -102,8 +105,8 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
             k[0] = h0A[s];
 #pragma unroll
             for (int d = 1; d < 27; ++d)
-                k[d] = pulled(hA[(d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]], d, c.w[WCLS[d]], dQu,
-                              dQd);
+                k[d] = pulled(ldnt(hA + (d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]), d, c.w[WCLS[d]],
+                              dQu, dQd);
         }
     }
 
@@ -262,14 +265,14 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
             od = gd;
             oe = ge;
         }
-        fB[(d - 1) * S + s] = od;
-        fB[(e - 1) * S + s] = oe;
+        stnt(od, fB + (d - 1) * S + s);
+        stnt(oe, fB + (e - 1) * S + s);
         if (z != nz - 1) {
-            hB[(d - 1) * S + s] = kd;
-            hB[(e - 1) * S + s] = ke;
+            stnt(kd, hB + (d - 1) * S + s);
+            stnt(ke, hB + (e - 1) * S + s);
             if (z == nz - 2) {
-                hB[(d - 1) * S + s + nxy] = kd;
-                hB[(e - 1) * S + s + nxy] = ke;
+                stnt(kd, hB + (d - 1) * S + s + nxy);
+                stnt(ke, hB + (e - 1) * S + s + nxy);
             }
         }
     }
@@ -305,13 +308,12 @@ hipError_t launch_lbm_f32s(const LbmParams& P, const PopBufs& pb, const MacroBuf
     c.hdy = (float)(0.5 / P.dy);
     c.hdz = (float)(0.5 / P.dz);
     const unsigned nblk = (P.S + 255) / 256;
-    const unsigned nblk_xcd = (nblk % 8 == 0) ? nblk / 8 : 0;
     dim3 g(nblk), b(256);
     int m = macro ? 1 : 0;
-    if (first && ebuf) lbm_f32s_kernel<true, true><<<g, b, 0, st>>>(P, c, pb, mb, m, nblk_xcd);
-    else if (first) lbm_f32s_kernel<true, false><<<g, b, 0, st>>>(P, c, pb, mb, m, nblk_xcd);
-    else if (ebuf) lbm_f32s_kernel<false, true><<<g, b, 0, st>>>(P, c, pb, mb, m, nblk_xcd);
-    else lbm_f32s_kernel<false, false><<<g, b, 0, st>>>(P, c, pb, mb, m, nblk_xcd);
+    if (first && ebuf) lbm_f32s_kernel<true, true><<<g, b, 0, st>>>(P, c, pb, mb, m);
+    else if (first) lbm_f32s_kernel<true, false><<<g, b, 0, st>>>(P, c, pb, mb, m);
+    else if (ebuf) lbm_f32s_kernel<false, true><<<g, b, 0, st>>>(P, c, pb, mb, m);
+    else lbm_f32s_kernel<false, false><<<g, b, 0, st>>>(P, c, pb, mb, m);
     return hipGetLastError();
 }
 

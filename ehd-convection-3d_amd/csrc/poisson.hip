@@ -222,6 +222,12 @@ __device__ __forceinline__ void bfly(cplx<T>* v, T sg)
     }
 }
 
+// LDS index with one pad element per 16: the stride-R stores of a Stockham
+// stage then spread over all banks instead of a quarter of them.
+__host__ __device__ constexpr int padi(int i) { return i + (i >> 4); }
+// padded LDS line length of the compile-time path, the plain length otherwise
+__host__ __device__ constexpr int lds_line(int N, int n) { return N > 0 ? padi(N - 1) + 1 : n; }
+
 template <typename T, int N, int ST, int NS>
 __device__ __forceinline__ cplx<T>* ct_stages(cplx<T>* in, cplx<T>* out, int nlines, int ldl,
                                               const cplx<T>* __restrict__ tw, int sign)
@@ -235,18 +241,19 @@ __device__ __forceinline__ cplx<T>* ct_stages(cplx<T>* in, cplx<T>* out, int nli
         const T sg = (T)sign;
         for (int w = threadIdx.x; w < nlines * NB; w += blockDim.x) {
             const int l = w / NB, j = w - l * NB, k = j % NS;
-            const cplx<T>* src = in + l * ldl + j;
-            cplx<T>* dst = out + l * ldl + (j / NS) * NS * R + k;
+            const cplx<T>* src = in + l * ldl;
+            cplx<T>* dst = out + l * ldl;
+            const int o = (j / NS) * NS * R + k;
             cplx<T> v[R];
-            v[0] = src[0];
+            v[0] = src[padi(j)];
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                if constexpr (NS == 1) v[r] = src[r * NB];
-                else v[r] = cmul(src[r * NB], root(tw, r * k * TSTEP, sign));
+                if constexpr (NS == 1) v[r] = src[padi(j + r * NB)];
+                else v[r] = cmul(src[padi(j + r * NB)], root(tw, r * k * TSTEP, sign));
             }
             bfly<T, R>(v, sg);
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[r * NS] = v[r];
+            for (int r = 0; r < R; ++r) dst[padi(o + r * NS)] = v[r];
         }
         __syncthreads();
         return ct_stages<T, N, ST + 1, NS * R>(out, in, nlines, ldl, tw, sign);
@@ -277,10 +284,11 @@ __global__ void __launch_bounds__(256) fftx_fwd_kernel(const T* __restrict__ q, 
 {
     const int nx = N > 0 ? N : nx_rt;
     const int nxh = nx / 2 + 1;
+    const int ldl = lds_line(N, nx);
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* tws = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* A = tws + nx;
-    cplx<T>* B = A + lpb * nx;
+    cplx<T>* B = A + lpb * ldl;
     for (int i = threadIdx.x; i < nx; i += blockDim.x) tws[i] = tw[i];
     const int l0 = blockIdx.x * lpb;
     const int nl = min(lpb, npairs - l0);
@@ -291,14 +299,15 @@ __global__ void __launch_bounds__(256) fftx_fwd_kernel(const T* __restrict__ q, 
         const int r0 = 2 * (l0 + l);
         const int z = r0 / ny + 1, y = r0 - (z - 1) * ny;
         const size_t o0 = (size_t)z * nxy + (size_t)y * nx + i;
-        A[l * nx + i] = {(T)((double)q[o0] * neps), (T)((double)q[o0 + nx] * neps)};
+        A[l * ldl + (N > 0 ? padi(i) : i)] = {(T)((double)q[o0] * neps), (T)((double)q[o0 + nx] * neps)};
     }
     __syncthreads();
-    cplx<T>* res = run_fft<T, N>(A, B, nx, nl, nx, pl, tws, -1);
+    cplx<T>* res = run_fft<T, N>(A, B, nx, nl, ldl, pl, tws, -1);
     for (int w = threadIdx.x; w < nl * nxh; w += blockDim.x) {
         const int l = w / nxh, k = w - l * nxh;
-        const cplx<T> zk = res[l * nx + k];
-        const cplx<T> zn = res[l * nx + ((nx - k) % nx)];
+        const int kn = (nx - k) % nx;
+        const cplx<T> zk = res[l * ldl + (N > 0 ? padi(k) : k)];
+        const cplx<T> zn = res[l * ldl + (N > 0 ? padi(kn) : kn)];
         // A_k = (Z_k + conj Z_{n-k})/2, B_k = (Z_k - conj Z_{n-k})/(2i)
         const cplx<T> ak = {(T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y)};
         const cplx<T> bk = {(T)0.5 * (zk.y + zn.y), (T)-0.5 * (zk.x - zn.x)};
@@ -315,10 +324,11 @@ __global__ void __launch_bounds__(256) ffty_kernel(cplx<T>* __restrict__ spec, c
                                                    FftPlan pl, int ny_rt, int nxh, int nstrips, int sign)
 {
     const int ny = N > 0 ? N : ny_rt;
+    const int ldl = lds_line(N, ny);
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* tws = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* A = tws + ny;
-    cplx<T>* B = A + W * ny;
+    cplx<T>* B = A + W * ldl;
     for (int i = threadIdx.x; i < ny; i += blockDim.x) tws[i] = tw[i];
     const int plane = blockIdx.x / nstrips;
     const int c0 = (blockIdx.x - plane * nstrips) * W;
@@ -326,13 +336,13 @@ __global__ void __launch_bounds__(256) ffty_kernel(cplx<T>* __restrict__ spec, c
     cplx<T>* base = spec + (size_t)plane * ny * nxh + c0;
     for (int w = threadIdx.x; w < W * ny; w += blockDim.x) {
         const int yy = w / W, c = w - yy * W;
-        if (c < nc) A[c * ny + yy] = base[(size_t)yy * nxh + c];
+        if (c < nc) A[c * ldl + (N > 0 ? padi(yy) : yy)] = base[(size_t)yy * nxh + c];
     }
     __syncthreads();
-    cplx<T>* res = run_fft<T, N>(A, B, ny, nc, ny, pl, tws, sign);
+    cplx<T>* res = run_fft<T, N>(A, B, ny, nc, ldl, pl, tws, sign);
     for (int w = threadIdx.x; w < W * ny; w += blockDim.x) {
         const int yy = w / W, c = w - yy * W;
-        if (c < nc) base[(size_t)yy * nxh + c] = res[c * ny + yy];
+        if (c < nc) base[(size_t)yy * nxh + c] = res[c * ldl + (N > 0 ? padi(yy) : yy)];
     }
 }
 
@@ -496,10 +506,11 @@ __global__ void __launch_bounds__(256) fftx_inv_kernel(const cplx<T>* __restrict
 {
     const int nx = N > 0 ? N : nx_rt;
     const int nxh = nx / 2 + 1;
+    const int ldl = lds_line(N, nx);
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* tws = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* A = tws + nx;
-    cplx<T>* B = A + lpb * nx;
+    cplx<T>* B = A + lpb * ldl;
     for (int i = threadIdx.x; i < nx; i += blockDim.x) tws[i] = tw[i];
     const int l0 = blockIdx.x * lpb;
     const int nl = min(lpb, npairs - l0);
@@ -518,17 +529,17 @@ __global__ void __launch_bounds__(256) fftx_inv_kernel(const cplx<T>* __restrict
             b.y = -b.y;
         }
         // Z = A + i B
-        A[l * nx + k] = {a.x - b.y, a.y + b.x};
+        A[l * ldl + (N > 0 ? padi(k) : k)] = {a.x - b.y, a.y + b.x};
     }
     __syncthreads();
-    cplx<T>* res = run_fft<T, N>(A, B, nx, nl, nx, pl, tws, +1);
+    cplx<T>* res = run_fft<T, N>(A, B, nx, nl, ldl, pl, tws, +1);
     const double inv = 1.0 / ((double)nx * (double)ny);
     const int nxy = nx * ny;
     for (int w = threadIdx.x; w < nl * nx; w += blockDim.x) {
         const int l = w / nx, i = w - l * nx;
         const int r0 = 2 * (l0 + l);
         const int z = r0 / ny + 1, y = r0 - (z - 1) * ny;
-        const cplx<T> v = res[l * nx + i];
+        const cplx<T> v = res[l * ldl + (N > 0 ? padi(i) : i)];
         double p0 = (double)v.x * inv, p1 = (double)v.y * inv;
         if (add_linear) {
             const double lin = voltage * (1.0 - (double)z / (double)(nz - 1));
@@ -576,16 +587,17 @@ hipError_t poisson_sized(const LbmParams& P, const PoissonBufs& pb, const T* q, 
     const FftPlan plx = make_fft_plan(nx), ply = make_fft_plan(ny);
     const size_t cs = sizeof(cplx<T>);
     // x passes: lpb complex lines (row pairs) per block, two buffers, ~32 KB
-    int lpb = (int)(32768 / (2 * nx * cs));
+    const int ldx = lds_line(NX, nx), ldy = lds_line(NY, ny);
+    int lpb = (int)(32768 / (2 * ldx * cs));
     if (lpb < 1) lpb = 1;
     if (lpb > npairs) lpb = npairs;
-    const size_t shx = (size_t)nx * cs + 2 * (size_t)lpb * nx * cs;
+    const size_t shx = (size_t)nx * cs + 2 * (size_t)lpb * ldx * cs;
     const int gx = (npairs + lpb - 1) / lpb;
     // y passes: strips of W columns, the widest (up to 16) whose buffers fit 64 KB
     int W = 16;
-    while (W > 1 && (size_t)ny * cs + 2 * (size_t)W * ny * cs > kMaxLds) W /= 2;
+    while (W > 2 && (size_t)ny * cs + 2 * (size_t)W * ldy * cs > kMaxLds) W /= 2;
     const int nstrips = (nxh + W - 1) / W;
-    const size_t shy = (size_t)ny * cs + 2 * (size_t)W * ny * cs;
+    const size_t shy = (size_t)ny * cs + 2 * (size_t)W * ldy * cs;
     if (shx > kMaxLds || shy > kMaxLds) return hipErrorInvalidValue;
     const int gy = nstrips * n;
 

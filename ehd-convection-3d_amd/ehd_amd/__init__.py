@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libehd_amd.so")
 
 FP64, FP32S = 0, 1
 INIT_REFERENCE, INIT_HYDROSTATIC, INIT_FIELDS = 0, 1, 2
+PERTURB_NOISE, PERTURB_ROLLS, PERTURB_SQUARES, PERTURB_CROSS, PERTURB_HEXAGONS = 0, 1, 2, 3, 4
 ERRORS = {0: "EHD_OK", -1: "EHD_EINVAL", -2: "EHD_EHIP", -3: "EHD_ENOMEM", -4: "EHD_ESTATE",
           -5: "EHD_ENONFINITE"}
 
@@ -39,6 +40,7 @@ class Params(C.Structure):
         ("lz", C.c_double), ("uw", C.c_double), ("exf", C.c_double),
         ("precision", C.c_int), ("init_mode", C.c_int),
         ("seed", C.c_uint64), ("perturb_amp", C.c_double), ("hydro_density", C.c_int),
+        ("perturb_pattern", C.c_int),
     ]
 
 
@@ -65,7 +67,7 @@ EXPORTS = (
     "ehd_init", "ehd_step", "ehd_lbm_step", "ehd_poisson", "ehd_read_fields", "ehd_write_phi",
     "ehd_write_charge", "ehd_read_populations", "ehd_write_populations", "ehd_diagnostics",
     "ehd_set_nusselt_reference", "ehd_profile_enable", "ehd_profile_read", "ehd_profile_reset",
-    "ehd_sync", "ehd_last_error", "ehd_abi_version",
+    "ehd_sync", "ehd_last_error", "ehd_abi_version", "ehd_save_state", "ehd_load_state",
 )
 
 _lib = None
@@ -102,6 +104,8 @@ def lib() -> C.CDLL:
     L.ehd_profile_read.argtypes = [vp, C.POINTER(Profile)]
     L.ehd_profile_reset.argtypes = [vp]
     L.ehd_sync.argtypes = [vp]
+    L.ehd_save_state.argtypes = [vp, C.c_char_p]
+    L.ehd_load_state.argtypes = [vp, C.c_char_p]
     L.ehd_last_error.argtypes = []
     L.ehd_last_error.restype = C.c_char_p
     L.ehd_abi_version.argtypes = []
@@ -134,7 +138,7 @@ def default_params() -> Params:
 def make_params(nx, ny, nz, T=200.0, C_=10.0, M=10.0, Fe=4000.0, Pr=10.0, lam=0.25, lam_c=1e-6,
                 precision=FP64, init_mode=INIT_HYDROSTATIC, seed=1, amp=1e-3,
                 cfl=0.01, voltage=1.0e4, eps=1.0e-4, rho0=1600.0, lz=1.0, uw=0.0, exf=0.0,
-                hydro_density=0) -> Params:
+                hydro_density=0, pattern=PERTURB_NOISE) -> Params:
     p = default_params()
     p.nx, p.ny, p.nz = nx, ny, nz
     p.T, p.C, p.M, p.Fe, p.Pr = T, C_, M, Fe, Pr
@@ -142,6 +146,7 @@ def make_params(nx, ny, nz, T=200.0, C_=10.0, M=10.0, Fe=4000.0, Pr=10.0, lam=0.
     p.cfl, p.voltage, p.eps, p.rho0, p.lz, p.uw, p.exf = cfl, voltage, eps, rho0, lz, uw, exf
     p.precision, p.init_mode, p.seed, p.perturb_amp = precision, init_mode, seed, amp
     p.hydro_density = hydro_density
+    p.perturb_pattern = pattern
     return p
 
 
@@ -248,3 +253,9 @@ class Box:
 
     def sync(self):
         _check(lib().ehd_sync(self._h))
+
+    def save_state(self, path: str):
+        _check(lib().ehd_save_state(self._h, os.fsencode(path)))
+
+    def load_state(self, path: str):
+        _check(lib().ehd_load_state(self._h, os.fsencode(path)))

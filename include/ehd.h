@@ -42,6 +42,15 @@ extern "C" {
                                   plus seeded uniform perturbation of u in the interior */
 #define EHD_INIT_FIELDS      2 /* caller-supplied rho,q,phi,u,v,w,Ex,Ey,Ez (read_data, LBM.cu:1596-1643) */
 
+/* perturbation of the velocity in EHD_INIT_HYDROSTATIC (amplitude perturb_amp);
+ * zeta = z/Lz, xi = x/Lx, eta = y/Ly */
+#define EHD_PERTURB_NOISE    0 /* seeded uniform noise U(-1,1) per node and component */
+#define EHD_PERTURB_ROLLS    1 /* x rolls: uz = (cos 2pi zeta - 1) cos 2pi xi, ux = Lx/Lz sin 2pi zeta sin 2pi xi
+                                  (the roll pattern of LBM.cu:391-394) */
+#define EHD_PERTURB_SQUARES  2 /* square cells (LBM.cu:396-399) */
+#define EHD_PERTURB_CROSS    3 /* superposed x and y rolls, the pattern LBM.cu:400-402 applies */
+#define EHD_PERTURB_HEXAGONS 4 /* hexagonal cells (LBM.cu:404-411), lateral scale L = Lx/sqrt(3) */
+
 /* Driver surface of main.cu (grid, T, C, M, Fe, Pr, TRT magic parameter) plus
  * the physical constants LBM.h fixes at compile time (LBM.h:31-80). */
 typedef struct ehd_params {
@@ -59,6 +68,7 @@ typedef struct ehd_params {
     double perturb_amp;       /* perturbation amplitude of u, physical units (1e-3) */
     int    hydro_density;     /* EHD_INIT_HYDROSTATIC: 0 = uniform rho0 (as gpu_initialization),
                                  1 = hydrostatic pressure, rho0 + (p - mean p)/cs^2 with dp/dz = q Ez */
+    int    perturb_pattern;   /* EHD_PERTURB_* (EHD_INIT_HYDROSTATIC) */
 } ehd_params;
 
 /* Dimensional constants derived from ehd_params (inverse of compute_parameters). */
@@ -134,6 +144,15 @@ int ehd_write_charge(ehd_box* b, const double* q);
 int ehd_read_populations(ehd_box* b, double* f0, double* f1, double* h0, double* h1);
 int ehd_write_populations(ehd_box* b, const double* f0, const double* f1,
                           const double* h0, const double* h1);
+
+/* Lossless checkpoint of the complete state (populations in their storage
+ * precision, potential, macroscopic fields, step and time).  The reference's
+ * restart file data_end.dat (LBM.cu:1505-1566, read back by read_data,
+ * LBM.cu:1596-1643) keeps 6 decimals of the macroscopic fields and
+ * re-equilibrates the populations; this one restarts bit for bit.  Loading
+ * needs a box created with the same grid and precision. */
+int ehd_save_state(ehd_box* b, const char* path);
+int ehd_load_state(ehd_box* b, const char* path);
 
 /* FP64 reductions on the device (current, total charge, umax, ...). */
 int ehd_diagnostics(ehd_box* b, ehd_diag* out);

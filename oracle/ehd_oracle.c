@@ -508,7 +508,54 @@ double oref_hydrostatic_e0(double C)
     return 0.5 * (lo + hi);
 }
 
+/* Velocity perturbation patterns of gpu_collide_save's perturb branch
+ * (LBM.cu:389-411): pattern 1 rolls (LBM.cu:394-395), 2 squares
+ * (LBM.cu:397-399), 3 crossed rolls (LBM.cu:400-402, the active form), 4
+ * hexagons (LBM.cu:404-411).  The reference's wavelength LL (LBM.h:39, 1.22)
+ * is the box width Lx of its 122-node grid; here the wavelength is the box
+ * width Lx (Ly for y), and the hexagon scale L = Lx/sqrt(3). */
+void oref_pattern_velocity(int pattern, double amp, double xx, double yy, double zz, double lx, double ly,
+                           double lz, double* u)
+{
+    const double pi = 3.14159265358979323846;
+    double zeta = zz / lz;
+    double cz = cos(2.0 * pi * zeta) - 1.0, sz = sin(2.0 * pi * zeta);
+    u[0] = u[1] = u[2] = 0.0;
+    switch (pattern) {
+    case 1:
+        u[2] = amp * cz * cos(2.0 * pi / lx * xx);
+        u[0] = amp * (lx / lz) * sz * sin(2.0 * pi / lx * xx);
+        break;
+    case 2:
+        u[2] = amp * cz * cos(2.0 * pi / lx * xx) * cos(2.0 * pi / ly * yy);
+        u[0] = 0.5 * amp * (lx / lz) * sz * sin(2.0 * pi / lx * xx) * cos(2.0 * pi / ly * yy);
+        u[1] = 0.5 * amp * (ly / lz) * sz * sin(2.0 * pi / ly * yy) * cos(2.0 * pi / lx * xx);
+        break;
+    case 3:
+        u[2] = amp * cz * (cos(2.0 * pi / lx * xx) + cos(2.0 * pi / ly * yy));
+        u[0] = amp * (lx / lz) * sz * sin(2.0 * pi / lx * xx);
+        u[1] = amp * (ly / lz) * sz * sin(2.0 * pi / ly * yy);
+        break;
+    case 4: {
+        double L = lx / sqrt(3.0), a = 4.0 * pi / 3.0 / L;
+        double kx = 2.0 * pi / (sqrt(3.0) * L), ky = 2.0 * pi / (3.0 * L);
+        u[2] = -amp * cz / 3.0 * (2.0 * cos(kx * xx) * cos(ky * yy) + cos(2.0 * ky * yy));
+        u[0] = -amp * 2.0 * pi * sz * 4.0 * pi / (3.0 * sqrt(3.0) * L * a * a) * sin(kx * xx) * cos(ky * yy);
+        u[1] = -amp * 2.0 * pi * sz * 4.0 * pi / (9.0 * L * a * a) * (cos(kx * xx) + 2.0 * cos(ky * yy))
+               * sin(ky * yy);
+        break;
+    }
+    default:
+        break;
+    }
+}
+
 void oref_init_hydrostatic(oref_box* b, uint64_t seed, double amp, int hydro_density)
+{
+    oref_init_hydrostatic_pattern(b, seed, amp, hydro_density, 0);
+}
+
+void oref_init_hydrostatic_pattern(oref_box* b, uint64_t seed, double amp, int hydro_density, int pattern)
 {
     const oref_params* p = &b->p;
     unsigned NX = p->nx, NY = p->ny, NZ = p->nz;
@@ -533,9 +580,17 @@ void oref_init_hydrostatic(oref_box* b, uint64_t seed, double amp, int hydro_den
                 b->fld[OREF_F_Q][s] = qz;
                 b->fld[OREF_F_PHI][s] = phz;
                 int wall = (z == 0 || z == NZ - 1);
-                b->fld[OREF_F_UX][s] = wall ? 0.0 : amp * oref_hash_uniform(seed, x, y, z, 0);
-                b->fld[OREF_F_UY][s] = wall ? 0.0 : amp * oref_hash_uniform(seed, x, y, z, 1);
-                b->fld[OREF_F_UZ][s] = wall ? 0.0 : amp * oref_hash_uniform(seed, x, y, z, 2);
+                double u[3];
+                if (pattern == 0) {
+                    u[0] = amp * oref_hash_uniform(seed, x, y, z, 0);
+                    u[1] = amp * oref_hash_uniform(seed, x, y, z, 1);
+                    u[2] = amp * oref_hash_uniform(seed, x, y, z, 2);
+                } else {
+                    oref_pattern_velocity(pattern, amp, p->dx * x, p->dy * y, p->dz * z, p->lx, p->ly, p->lz, u);
+                }
+                b->fld[OREF_F_UX][s] = wall ? 0.0 : u[0];
+                b->fld[OREF_F_UY][s] = wall ? 0.0 : u[1];
+                b->fld[OREF_F_UZ][s] = wall ? 0.0 : u[2];
             }
     }
     oref_efield(b);
@@ -1057,6 +1112,15 @@ void oref_step(oref_box* b, int nsteps)
         oref_stream_collide_save(b);   /* main.cu:184 */
         oref_poisson(b);               /* main.cu:191-203 */
         oref_efield(b);                /* main.cu:206 */
+    }
+}
+
+void oref_step_tridiag(oref_box* b, int nsteps)
+{
+    for (int i = 0; i < nsteps; ++i) {
+        oref_stream_collide_save(b);
+        oref_poisson_tridiag(b);
+        oref_efield(b);
     }
 }
 

@@ -73,6 +73,11 @@ void oref_set_fields(oref_box* b, const double* f9);   /* read_data semantics (L
  * interior; hydro_density = 1 also sets the hydrostatic pressure (rho0 +
  * (p - mean p)/cs^2, dp/dz = q Ez), 0 keeps rho = rho0 (gpu_initialization). */
 void oref_init_hydrostatic(oref_box* b, uint64_t seed, double amp, int hydro_density);
+/* as above with the velocity perturbation pattern (0 noise, 1 rolls, 2 squares,
+ * 3 crossed rolls, 4 hexagons) */
+void oref_init_hydrostatic_pattern(oref_box* b, uint64_t seed, double amp, int hydro_density, int pattern);
+void oref_pattern_velocity(int pattern, double amp, double xx, double yy, double zz, double lx, double ly,
+                           double lz, double* u);
 void oref_init_equilibrium(oref_box* b);               /* LBM.cu:98-276 */
 
 /* LBM phases (LBM.cu:278-1353). */
@@ -95,6 +100,10 @@ void oref_poisson_tridiag(oref_box* b);
 
 /* One coupled step = main.cu:184-208; n steps. */
 void oref_step(oref_box* b, int nsteps);
+/* The same step with oref_poisson_tridiag as the Poisson solve: two FP64
+ * evaluations of one discrete problem, used to measure the FP64 floor of the
+ * coupled dynamics (how far rounding-level differences in phi carry). */
+void oref_step_tridiag(oref_box* b, int nsteps);
 
 /* Collector current (LBM.cu:1646-1671) on copies of q and Ez (box unchanged). */
 double oref_current(oref_box* b);

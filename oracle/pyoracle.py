@@ -70,8 +70,13 @@ def lib() -> C.CDLL:
             getattr(L, name).restype = None
         L.oref_set_fields.argtypes = [vp, dp]
         L.oref_init_hydrostatic.argtypes = [vp, C.c_uint64, C.c_double, C.c_int]
+        L.oref_init_hydrostatic_pattern.argtypes = [vp, C.c_uint64, C.c_double, C.c_int, C.c_int]
+        L.oref_init_hydrostatic_pattern.restype = None
+        L.oref_pattern_velocity.argtypes = [C.c_int] + [C.c_double] * 7 + [dp]
+        L.oref_pattern_velocity.restype = None
         L.oref_fft3d.argtypes = [vp, dp, dp, C.c_int]
         L.oref_step.argtypes = [vp, C.c_int]
+        L.oref_step_tridiag.argtypes = [vp, C.c_int]
         L.oref_current.argtypes = [vp]
         L.oref_current.restype = C.c_double
         L.oref_fft1d.argtypes = [dp, C.c_int, C.c_int]
@@ -175,8 +180,8 @@ class Box:
     # phases
     def initialization(self): lib().oref_initialization(self._b)
     def init_equilibrium(self): lib().oref_init_equilibrium(self._b)
-    def init_hydrostatic(self, seed: int, amp: float, hydro_density: int = 0):
-        lib().oref_init_hydrostatic(self._b, seed, amp, int(hydro_density))
+    def init_hydrostatic(self, seed: int, amp: float, hydro_density: int = 0, pattern: int = 0):
+        lib().oref_init_hydrostatic_pattern(self._b, seed, amp, int(hydro_density), int(pattern))
 
     def set_fields(self, f9: np.ndarray):
         a = np.ascontiguousarray(f9, dtype=np.float64)
@@ -195,6 +200,7 @@ class Box:
     def poisson(self): lib().oref_poisson(self._b)
     def poisson_tridiag(self): lib().oref_poisson_tridiag(self._b)
     def step(self, n: int = 1): lib().oref_step(self._b, int(n))
+    def step_tridiag(self, n: int = 1): lib().oref_step_tridiag(self._b, int(n))
     def current(self) -> float: return lib().oref_current(self._b)
 
     def rates(self):
@@ -205,3 +211,11 @@ class Box:
     def fft3d(self, src: int, dst: int, sign: int):
         dp = C.POINTER(C.c_double)
         lib().oref_fft3d(self._b, self[src].ctypes.data_as(dp), self[dst].ctypes.data_as(dp), int(sign))
+
+
+def pattern_velocity(pattern: int, amp: float, xx: float, yy: float, zz: float, lx: float, ly: float,
+                     lz: float) -> np.ndarray:
+    """(ux, uy, uz) of a perturbation pattern at one physical point."""
+    u = np.zeros(3)
+    lib().oref_pattern_velocity(int(pattern), amp, xx, yy, zz, lx, ly, lz, u.ctypes.data_as(C.POINTER(C.c_double)))
+    return u

@@ -412,6 +412,43 @@ def test_hash_perturbation():
     assert O.hash_uniform(1, 3, 4, 5, 2) != O.hash_uniform(2, 3, 4, 5, 2)
 
 
+@pytest.mark.parametrize("pattern", [1, 2, 3, 4])
+def test_perturbation_patterns_divergence_free(pattern):
+    """Every pattern of LBM.cu:389-411 is an incompressible cell: div u = 0
+    (central differences of the analytic field), u = 0 on both walls, and the
+    roll/square/cross fields repeat with the box width.  A wrong sign, a
+    swapped component or a wrong wavenumber breaks one of these."""
+    lx, ly, lz, amp = 1.22, 1.22 * (np.sqrt(3.0) if pattern == 4 else 1.0), 1.0, 1e-3
+    rng = np.random.default_rng(pattern)
+    h = 1e-5
+    scale = 0.0
+    for _ in range(40):
+        x, y, z = rng.uniform(0, lx), rng.uniform(0, ly), rng.uniform(0.05, 0.95) * lz
+        u = lambda a, b, c: O.pattern_velocity(pattern, amp, a, b, c, lx, ly, lz)
+        div = ((u(x + h, y, z)[0] - u(x - h, y, z)[0]) + (u(x, y + h, z)[1] - u(x, y - h, z)[1])
+               + (u(x, y, z + h)[2] - u(x, y, z - h)[2])) / (2 * h)
+        grad = np.abs(u(x, y, z + h)[2] - u(x, y, z - h)[2]) / (2 * h)
+        scale = max(scale, grad)
+        assert abs(div) < 1e-7 * amp, (pattern, div)
+        for zw in (0.0, lz):
+            assert np.allclose(u(x, y, zw), 0.0, atol=1e-15)
+        assert np.allclose(u(x + lx, y, z), u(x, y, z), atol=1e-13)
+        assert np.allclose(u(x, y + ly, z), u(x, y, z), atol=1e-13)
+    assert scale > 1e-3 * amp   # the field is not trivially zero
+
+
+def test_crossed_rolls_reference_form():
+    """Pattern 3 equals the active perturbation of LBM.cu:400-402 with LL = Lx
+    (LBM.h:39 sets LL = 1.22, the width of the 122-node box)."""
+    LL = 1.22
+    for x, y, z in ((0.1, 0.3, 0.2), (0.9, 1.1, 0.7), (0.5, 0.05, 0.5)):
+        u = O.pattern_velocity(3, 0.001, x, y, z, LL, LL, 1.0)
+        uz = (np.cos(2 * np.pi * z) - 1) * (np.cos(2 * np.pi / LL * x) + np.cos(2 * np.pi / LL * y)) * 0.001
+        ux = LL * np.sin(2 * np.pi * z) * np.sin(2 * np.pi / LL * x) * 0.001
+        uy = LL * np.sin(2 * np.pi * z) * np.sin(2 * np.pi / LL * y) * 0.001
+        assert np.allclose(u, [ux, uy, uz], rtol=1e-14, atol=1e-18)
+
+
 def test_init_hydrostatic_state():
     b = _box(6, 4, 11, C_=10.0)
     b.init_hydrostatic(3, 1e-3, hydro_density=1)
