@@ -13,14 +13,15 @@
 
 #include "ehd_internal.h"
 #include "fft_plan.h"
+#include "ehd_error.h"
 
 using namespace ehd;
 
 namespace {
-
 thread_local char g_err[512] = "";
+}  // namespace
 
-int fail(int code, const char* fmt, ...)
+int ehd::set_error(int code, const char* fmt, ...)
 {
     va_list ap;
     va_start(ap, fmt);
@@ -29,11 +30,14 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
+namespace {
+
+
 #define HIPCHK(call)                                                                          \
     do {                                                                                      \
         hipError_t e_ = (call);                                                               \
         if (e_ != hipSuccess)                                                                 \
-            return fail(EHD_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+            return set_error(EHD_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
     } while (0)
 
 constexpr double kPi = 3.14159265358979323846;
@@ -117,21 +121,21 @@ int resolve_events(ehd_box* b)
 
 int validate(const ehd_params* p)
 {
-    if (!p) return fail(EHD_EINVAL, "null params");
+    if (!p) return set_error(EHD_EINVAL, "null params");
     if (p->nx < 2 || p->ny < 2 || (p->nx & 1) || (p->ny & 1))
-        return fail(EHD_EINVAL, "nx and ny must be even and >= 2 (got %d, %d)", p->nx, p->ny);
-    if (p->nz < 4) return fail(EHD_EINVAL, "nz must be >= 4 (got %d)", p->nz);
+        return set_error(EHD_EINVAL, "nx and ny must be even and >= 2 (got %d, %d)", p->nx, p->ny);
+    if (p->nz < 4) return set_error(EHD_EINVAL, "nz must be >= 4 (got %d)", p->nz);
     const double S = (double)p->nx * p->ny * p->nz;
-    if (S * 27.0 >= 4294967295.0) return fail(EHD_EINVAL, "grid too large: 27*nx*ny*nz must fit 32 bits");
+    if (S * 27.0 >= 4294967295.0) return set_error(EHD_EINVAL, "grid too large: 27*nx*ny*nz must fit 32 bits");
     if (!(p->T > 0) || !(p->C > 0) || !(p->M > 0) || !(p->Fe > 0))
-        return fail(EHD_EINVAL, "T, C, M, Fe must be positive");
+        return set_error(EHD_EINVAL, "T, C, M, Fe must be positive");
     if (!(p->cfl > 0) || !(p->voltage > 0) || !(p->eps > 0) || !(p->rho0 > 0) || !(p->lz > 0))
-        return fail(EHD_EINVAL, "cfl, voltage, eps, rho0, lz must be positive");
-    if (!(p->lambda_f > 0) || !(p->lambda_c > 0)) return fail(EHD_EINVAL, "TRT magic parameters must be positive");
-    if (p->precision != EHD_FP64 && p->precision != EHD_FP32S) return fail(EHD_EINVAL, "unknown precision");
-    if (p->init_mode < EHD_INIT_REFERENCE || p->init_mode > EHD_INIT_FIELDS) return fail(EHD_EINVAL, "unknown init mode");
+        return set_error(EHD_EINVAL, "cfl, voltage, eps, rho0, lz must be positive");
+    if (!(p->lambda_f > 0) || !(p->lambda_c > 0)) return set_error(EHD_EINVAL, "TRT magic parameters must be positive");
+    if (p->precision != EHD_FP64 && p->precision != EHD_FP32S) return set_error(EHD_EINVAL, "unknown precision");
+    if (p->init_mode < EHD_INIT_REFERENCE || p->init_mode > EHD_INIT_FIELDS) return set_error(EHD_EINVAL, "unknown init mode");
     if (p->perturb_pattern < EHD_PERTURB_NOISE || p->perturb_pattern > EHD_PERTURB_HEXAGONS)
-        return fail(EHD_EINVAL, "unknown perturbation pattern");
+        return set_error(EHD_EINVAL, "unknown perturbation pattern");
     return EHD_OK;
 }
 
@@ -176,7 +180,7 @@ int alloc(void** p, size_t bytes)
 {
     if (hipMalloc(p, bytes) != hipSuccess) {
         *p = nullptr;
-        return fail(EHD_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+        return set_error(EHD_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
     }
     return EHD_OK;
 }
@@ -213,13 +217,13 @@ int run_lbm(ehd_box* b, bool macro)
     if (b->prof) {
         e0 = take_event(b);
         e1 = take_event(b);
-        if (!e0 || !e1) return fail(EHD_EHIP, "hipEventCreate failed");
+        if (!e0 || !e1) return set_error(EHD_EHIP, "hipEventCreate failed");
         HIPCHK(hipEventRecord(e0, b->st));
     }
     hipError_t e;
     if (b->prec == Prec::F64) e = launch_lbm_f64(b->lp, pop_bufs(b), macro_bufs(b), b->direct, b->ebuf, macro, b->st);
     else e = launch_lbm_f32s(b->lp, pop_bufs(b), macro_bufs(b), b->direct, b->ebuf, macro, b->st);
-    if (e != hipSuccess) return fail(EHD_EHIP, "LBM launch failed: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return set_error(EHD_EHIP, "LBM launch failed: %s", hipGetErrorString(e));
     if (b->prof) {
         HIPCHK(hipEventRecord(e1, b->st));
         b->lbm_pairs.emplace_back(e0, e1);
@@ -235,12 +239,12 @@ int run_poisson(ehd_box* b)
     if (b->prof) {
         e0 = take_event(b);
         e1 = take_event(b);
-        if (!e0 || !e1) return fail(EHD_EHIP, "hipEventCreate failed");
+        if (!e0 || !e1) return set_error(EHD_EHIP, "hipEventCreate failed");
         HIPCHK(hipEventRecord(e0, b->st));
     }
     PoissonBufs pb{b->nx, b->ny, b->nz, b->nxh, b->spec, b->twx, b->twy, b->phibar};
     hipError_t e = launch_poisson(b->prec, b->lp, pb, b->q, b->phi, b->prm.voltage, b->prm.eps, b->st);
-    if (e != hipSuccess) return fail(EHD_EHIP, "Poisson launch failed: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return set_error(EHD_EHIP, "Poisson launch failed: %s", hipGetErrorString(e));
     if (b->prof) {
         HIPCHK(hipEventRecord(e1, b->st));
         b->poi_pairs.emplace_back(e0, e1);
@@ -275,7 +279,7 @@ const char* ehd_last_error(void) { return g_err; }
 
 int ehd_params_default(ehd_params* p)
 {
-    if (!p) return fail(EHD_EINVAL, "null params");
+    if (!p) return set_error(EHD_EINVAL, "null params");
     std::memset(p, 0, sizeof(*p));
     p->nx = 122; p->ny = 122; p->nz = 101;                     // LBM.h:34-36
     const double K = 2.5e-5, eps = 1.0e-4, V = 1.0e4, nu = 0.0251004, rho0 = 1600.0, q0 = 10.0,
@@ -299,7 +303,7 @@ int ehd_derive(const ehd_params* p, ehd_derived* d)
 {
     int rc = validate(p);
     if (rc) return rc;
-    if (!d) return fail(EHD_EINVAL, "null output");
+    if (!d) return set_error(EHD_EINVAL, "null output");
     std::memset(d, 0, sizeof(*d));
     d->dz = p->lz / (double)(p->nz - 1);
     d->dx = d->dz;
@@ -323,7 +327,7 @@ int ehd_derive(const ehd_params* p, ehd_derived* d)
 
 int ehd_compute_parameters(const ehd_derived* d, const ehd_params* p, double* T, double* M, double* C, double* Fe)
 {
-    if (!d || !p || !T || !M || !C || !Fe) return fail(EHD_EINVAL, "null argument");
+    if (!d || !p || !T || !M || !C || !Fe) return set_error(EHD_EINVAL, "null argument");
     *M = std::sqrt(p->eps / p->rho0) / d->K;     // LBM.cu:1375-1378
     *T = p->eps * p->voltage / d->K / d->nu / p->rho0;
     *C = d->charge0 * p->lz * p->lz / (p->voltage * p->eps);
@@ -333,12 +337,12 @@ int ehd_compute_parameters(const ehd_derived* d, const ehd_params* p, double* T,
 
 int ehd_create(const ehd_params* p, ehd_box** out)
 {
-    if (!out) return fail(EHD_EINVAL, "null output");
+    if (!out) return set_error(EHD_EINVAL, "null output");
     *out = nullptr;
     int rc = validate(p);
     if (rc) return rc;
     ehd_box* b = new (std::nothrow) ehd_box();
-    if (!b) return fail(EHD_ENOMEM, "host allocation failed");
+    if (!b) return set_error(EHD_ENOMEM, "host allocation failed");
     b->prm = *p;
     ehd_derive(p, &b->der);
     b->prec = (p->precision == EHD_FP64) ? Prec::F64 : Prec::F32S;
@@ -354,7 +358,7 @@ int ehd_create(const ehd_params* p, ehd_box** out)
         return code;
     };
     if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(EHD_EHIP, "hipStreamCreate failed: %s", hipGetErrorString(hipGetLastError())));
+        return bail(set_error(EHD_EHIP, "hipStreamCreate failed: %s", hipGetErrorString(hipGetLastError())));
     const size_t S = b->S, es = b->esz;
     for (int i = 0; i < 2; ++i) {
         if ((rc = alloc(&b->f[i], 26 * S * es)) || (rc = alloc(&b->f0[i], S * es)) ||
@@ -405,7 +409,7 @@ int ehd_create(const ehd_params* p, ehd_box** out)
         if (hipMemcpy(b->shift_rs, rs.data(), nz * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(b->shift_qs, qs.data(), nz * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(b->shift_f, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
-            return bail(fail(EHD_EHIP, "shift table upload failed"));
+            return bail(set_error(EHD_EHIP, "shift table upload failed"));
         b->lp.rs = b->shift_rs;
         b->lp.qs = b->shift_qs;
         b->lp.shf = b->shift_f;
@@ -417,14 +421,14 @@ int ehd_create(const ehd_params* p, ehd_box** out)
         make_twiddles(b->ny, ty);
         if (hipMemcpy(b->twx, tx.data(), tx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(b->twy, ty.data(), ty.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
-            return bail(fail(EHD_EHIP, "twiddle upload failed"));
+            return bail(set_error(EHD_EHIP, "twiddle upload failed"));
     } else {
         std::vector<float> tx, ty;
         make_twiddles(b->nx, tx);
         make_twiddles(b->ny, ty);
         if (hipMemcpy(b->twx, tx.data(), tx.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(b->twy, ty.data(), ty.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
-            return bail(fail(EHD_EHIP, "twiddle upload failed"));
+            return bail(set_error(EHD_EHIP, "twiddle upload failed"));
     }
     *out = b;
     return EHD_OK;
@@ -440,10 +444,10 @@ void ehd_destroy(ehd_box* b)
 
 int ehd_init(ehd_box* b, const double* fields9)
 {
-    if (!b) return fail(EHD_EINVAL, "null box");
+    if (!b) return set_error(EHD_EINVAL, "null box");
     const ehd_params& p = b->prm;
     const size_t S = b->S;
-    if (p.init_mode == EHD_INIT_FIELDS && !fields9) return fail(EHD_EINVAL, "EHD_INIT_FIELDS needs fields");
+    if (p.init_mode == EHD_INIT_FIELDS && !fields9) return set_error(EHD_EINVAL, "EHD_INIT_FIELDS needs fields");
     // double macroscopic fields rho,q,phi,u,v,w on the device
     double* tmp = nullptr;
     int rc = alloc((void**)&tmp, 6 * S * sizeof(double));
@@ -488,7 +492,7 @@ int ehd_init(ehd_box* b, const double* fields9)
         hipError_t e4 = hipStreamSynchronize(b->st);
         (void)hipFree(dqz);
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess)
-            return fail(EHD_EHIP, "hydrostatic init failed");
+            return set_error(EHD_EHIP, "hydrostatic init failed");
     } else {
         HIPCHK(hipMemcpyAsync(rho, fields9 + 0 * S, S * 8, hipMemcpyHostToDevice, b->st));
         HIPCHK(hipMemcpyAsync(q, fields9 + 1 * S, S * 8, hipMemcpyHostToDevice, b->st));
@@ -522,14 +526,14 @@ int ehd_init(ehd_box* b, const double* fields9)
 
 int ehd_step(ehd_box* b, int nsteps)
 {
-    if (!b) return fail(EHD_EINVAL, "null box");
-    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
-    if (nsteps < 0) return fail(EHD_EINVAL, "nsteps < 0");
+    if (!b) return set_error(EHD_EINVAL, "null box");
+    if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
+    if (nsteps < 0) return set_error(EHD_EINVAL, "nsteps < 0");
     hipEvent_t s0 = nullptr, s1 = nullptr;
     if (b->prof && nsteps > 0) {
         s0 = take_event(b);
         s1 = take_event(b);
-        if (!s0 || !s1) return fail(EHD_EHIP, "hipEventCreate failed");
+        if (!s0 || !s1) return set_error(EHD_EHIP, "hipEventCreate failed");
         HIPCHK(hipEventRecord(s0, b->st));
     }
     for (int i = 0; i < nsteps; ++i) {
@@ -551,23 +555,23 @@ int ehd_step(ehd_box* b, int nsteps)
 
 int ehd_lbm_step(ehd_box* b)
 {
-    if (!b) return fail(EHD_EINVAL, "null box");
-    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    if (!b) return set_error(EHD_EINVAL, "null box");
+    if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
     return run_lbm(b, true);
 }
 
 int ehd_poisson(ehd_box* b)
 {
-    if (!b) return fail(EHD_EINVAL, "null box");
-    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    if (!b) return set_error(EHD_EINVAL, "null box");
+    if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
     return run_poisson(b);
 }
 
 int ehd_read_fields(ehd_box* b, double* rho, double* q, double* phi, double* ux, double* uy, double* uz, double* ex,
                     double* ey, double* ez)
 {
-    if (!b) return fail(EHD_EINVAL, "null box");
-    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    if (!b) return set_error(EHD_EINVAL, "null box");
+    if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
     const size_t S = b->S;
     double* tmp = nullptr;
     int rc = alloc((void**)&tmp, S * sizeof(double));
@@ -603,8 +607,8 @@ int ehd_read_fields(ehd_box* b, double* rho, double* q, double* phi, double* ux,
 
 int ehd_write_phi(ehd_box* b, const double* phi)
 {
-    if (!b || !phi) return fail(EHD_EINVAL, "null argument");
-    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    if (!b || !phi) return set_error(EHD_EINVAL, "null argument");
+    if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
     double* tmp = nullptr;
     int rc = alloc((void**)&tmp, b->S * sizeof(double));
     if (rc) return rc;
@@ -621,8 +625,8 @@ int ehd_write_phi(ehd_box* b, const double* phi)
 
 int ehd_write_charge(ehd_box* b, const double* q)
 {
-    if (!b || !q) return fail(EHD_EINVAL, "null argument");
-    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    if (!b || !q) return set_error(EHD_EINVAL, "null argument");
+    if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
     double* tmp = nullptr;
     int rc = alloc((void**)&tmp, b->S * sizeof(double));
     if (rc) return rc;
@@ -638,8 +642,8 @@ int ehd_write_charge(ehd_box* b, const double* q)
 
 int ehd_read_populations(ehd_box* b, double* f0, double* f1, double* h0, double* h1)
 {
-    if (!b) return fail(EHD_EINVAL, "null box");
-    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    if (!b) return set_error(EHD_EINVAL, "null box");
+    if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
     const size_t S = b->S;
     double* tmp = nullptr;
     int rc = alloc((void**)&tmp, 54 * S * sizeof(double));
@@ -662,8 +666,8 @@ int ehd_read_populations(ehd_box* b, double* f0, double* f1, double* h0, double*
 
 int ehd_write_populations(ehd_box* b, const double* f0, const double* f1, const double* h0, const double* h1)
 {
-    if (!b || !f0 || !f1 || !h0 || !h1) return fail(EHD_EINVAL, "null argument");
-    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    if (!b || !f0 || !f1 || !h0 || !h1) return set_error(EHD_EINVAL, "null argument");
+    if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
     const size_t S = b->S;
     double* tmp = nullptr;
     int rc = alloc((void**)&tmp, 54 * S * sizeof(double));
@@ -714,9 +718,9 @@ int ckpt_io(ehd_box* b, FILE* f, bool save)
         buf.resize(it.bytes);
         if (save) {
             HIPCHK(hipMemcpy(buf.data(), it.dev, it.bytes, hipMemcpyDeviceToHost));
-            if (std::fwrite(buf.data(), 1, it.bytes, f) != it.bytes) return fail(EHD_EINVAL, "checkpoint write failed");
+            if (std::fwrite(buf.data(), 1, it.bytes, f) != it.bytes) return set_error(EHD_EINVAL, "checkpoint write failed");
         } else {
-            if (std::fread(buf.data(), 1, it.bytes, f) != it.bytes) return fail(EHD_EINVAL, "checkpoint truncated");
+            if (std::fread(buf.data(), 1, it.bytes, f) != it.bytes) return set_error(EHD_EINVAL, "checkpoint truncated");
             HIPCHK(hipMemcpy(it.dev, buf.data(), it.bytes, hipMemcpyHostToDevice));
         }
     }
@@ -726,11 +730,11 @@ int ckpt_io(ehd_box* b, FILE* f, bool save)
 
 int ehd_save_state(ehd_box* b, const char* path)
 {
-    if (!b || !path) return fail(EHD_EINVAL, "null argument");
-    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    if (!b || !path) return set_error(EHD_EINVAL, "null argument");
+    if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
     HIPCHK(hipStreamSynchronize(b->st));
     FILE* f = std::fopen(path, "wb");
-    if (!f) return fail(EHD_EINVAL, "cannot open %s for writing", path);
+    if (!f) return set_error(EHD_EINVAL, "cannot open %s for writing", path);
     CkptHeader h{};
     std::memcpy(h.magic, "EHDCKPT1", 8);
     h.abi = EHD_ABI_VERSION;
@@ -742,24 +746,24 @@ int ehd_save_state(ehd_box* b, const char* path)
     h.t = b->t;
     h.T = b->prm.T; h.C = b->prm.C; h.M = b->prm.M; h.Fe = b->prm.Fe;
     h.lambda_f = b->prm.lambda_f; h.lambda_c = b->prm.lambda_c;
-    int rc = (std::fwrite(&h, sizeof(h), 1, f) == 1) ? ckpt_io(b, f, true) : fail(EHD_EINVAL, "checkpoint write failed");
-    if (std::fclose(f) != 0 && rc == EHD_OK) rc = fail(EHD_EINVAL, "checkpoint close failed");
+    int rc = (std::fwrite(&h, sizeof(h), 1, f) == 1) ? ckpt_io(b, f, true) : set_error(EHD_EINVAL, "checkpoint write failed");
+    if (std::fclose(f) != 0 && rc == EHD_OK) rc = set_error(EHD_EINVAL, "checkpoint close failed");
     return rc;
 }
 
 int ehd_load_state(ehd_box* b, const char* path)
 {
-    if (!b || !path) return fail(EHD_EINVAL, "null argument");
+    if (!b || !path) return set_error(EHD_EINVAL, "null argument");
     FILE* f = std::fopen(path, "rb");
-    if (!f) return fail(EHD_EINVAL, "cannot open %s", path);
+    if (!f) return set_error(EHD_EINVAL, "cannot open %s", path);
     CkptHeader h{};
     if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, "EHDCKPT1", 8) != 0) {
         std::fclose(f);
-        return fail(EHD_EINVAL, "%s is not a checkpoint", path);
+        return set_error(EHD_EINVAL, "%s is not a checkpoint", path);
     }
     if (h.precision != b->prm.precision || h.nx != b->nx || h.ny != b->ny || h.nz != b->nz) {
         std::fclose(f);
-        return fail(EHD_EINVAL, "checkpoint grid/precision (%dx%dx%d, %d) differ from the box", h.nx, h.ny, h.nz,
+        return set_error(EHD_EINVAL, "checkpoint grid/precision (%dx%dx%d, %d) differ from the box", h.nx, h.ny, h.nz,
                     h.precision);
     }
     HIPCHK(hipStreamSynchronize(b->st));
@@ -776,15 +780,15 @@ int ehd_load_state(ehd_box* b, const char* path)
 
 int ehd_set_nusselt_reference(ehd_box* b, double current_hs)
 {
-    if (!b) return fail(EHD_EINVAL, "null box");
+    if (!b) return set_error(EHD_EINVAL, "null box");
     b->current_hs = current_hs;
     return EHD_OK;
 }
 
 int ehd_diagnostics(ehd_box* b, ehd_diag* out)
 {
-    if (!b || !out) return fail(EHD_EINVAL, "null argument");
-    if (!b->initialised) return fail(EHD_ESTATE, "box not initialised");
+    if (!b || !out) return set_error(EHD_EINVAL, "null argument");
+    if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
     HIPCHK(launch_diag(b->prec, b->lp, b->q, b->uz, b->rho, b->phi, b->phibar, b->diag_scr, b->diag_out, b->st));
     double r[8] = {0};
     HIPCHK(hipMemcpyAsync(r, b->diag_out, 5 * sizeof(double), hipMemcpyDeviceToHost, b->st));
@@ -797,20 +801,20 @@ int ehd_diagnostics(ehd_box* b, ehd_diag* out)
     out->nusselt = (b->current_hs != 0.0) ? out->current / b->current_hs : 0.0;
     out->step = b->step;
     out->time = b->t;
-    if (r[4] != 0.0) return fail(EHD_ENONFINITE, "non-finite charge or velocity at %.0f nodes (step %lld)", r[4], b->step);
+    if (r[4] != 0.0) return set_error(EHD_ENONFINITE, "non-finite charge or velocity at %.0f nodes (step %lld)", r[4], b->step);
     return EHD_OK;
 }
 
 int ehd_profile_enable(ehd_box* b, int on)
 {
-    if (!b) return fail(EHD_EINVAL, "null box");
+    if (!b) return set_error(EHD_EINVAL, "null box");
     b->prof = (on != 0);
     return EHD_OK;
 }
 
 int ehd_profile_read(ehd_box* b, ehd_profile* out)
 {
-    if (!b || !out) return fail(EHD_EINVAL, "null argument");
+    if (!b || !out) return set_error(EHD_EINVAL, "null argument");
     int rc = resolve_events(b);
     if (rc) return rc;
     out->lbm_ms = b->lbm_ms;
@@ -824,7 +828,7 @@ int ehd_profile_read(ehd_box* b, ehd_profile* out)
 
 int ehd_profile_reset(ehd_box* b)
 {
-    if (!b) return fail(EHD_EINVAL, "null box");
+    if (!b) return set_error(EHD_EINVAL, "null box");
     int rc = resolve_events(b);
     if (rc) return rc;
     b->lbm_ms = b->poi_ms = b->step_ms = 0.0;
@@ -834,7 +838,7 @@ int ehd_profile_reset(ehd_box* b)
 
 int ehd_sync(ehd_box* b)
 {
-    if (!b) return fail(EHD_EINVAL, "null box");
+    if (!b) return set_error(EHD_EINVAL, "null box");
     HIPCHK(hipStreamSynchronize(b->st));
     return EHD_OK;
 }

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "ehd.h"
+#include "ehd_io.h"
 
 namespace {
 
@@ -120,112 +121,21 @@ void die(int rc, const char* what)
         if (rc_ != EHD_OK) die(rc_, #x); \
     } while (0)
 
+// the nine fields of the box on the host, in the order of ehd_io.h
 struct Fields {
-    int nx, ny, nz;
-    std::vector<double> rho, q, phi, ux, uy, uz, ex, ey, ez;
-    explicit Fields(const ehd_params& p) : nx(p.nx), ny(p.ny), nz(p.nz)
-    {
-        const size_t S = (size_t)nx * ny * nz;
-        for (auto* v : {&rho, &q, &phi, &ux, &uy, &uz, &ex, &ey, &ez}) v->assign(S, 0.0);
-    }
-    size_t idx(int x, int y, int z) const { return (size_t)nx * ((size_t)ny * z + y) + x; }
+    size_t S;
+    std::vector<double> f9;
+    explicit Fields(const ehd_params& p) : S((size_t)p.nx * p.ny * p.nz), f9(9 * S, 0.0) {}
+    double* at(int k) { return f9.data() + (size_t)k * S; }
     void read(ehd_box* b)
     {
-        CALL(ehd_read_fields(b, rho.data(), q.data(), phi.data(), ux.data(), uy.data(), uz.data(), ex.data(),
-                             ey.data(), ez.data()));
-    }
-    // linear extrapolation of the wall rows (LBM.cu:1457-1472)
-    void extrapolate(std::vector<double>& v, bool bottom, bool top) const
-    {
-        for (int y = 0; y < ny; ++y)
-            for (int x = 0; x < nx; ++x) {
-                if (bottom) v[idx(x, y, 0)] = 2.0 * v[idx(x, y, 1)] - v[idx(x, y, 2)];
-                if (top) v[idx(x, y, nz - 1)] = 2.0 * v[idx(x, y, nz - 2)] - v[idx(x, y, nz - 3)];
-            }
+        CALL(ehd_read_fields(b, at(0), at(1), at(2), at(3), at(4), at(5), at(6), at(7), at(8)));
     }
 };
 
-// save_data_tecplot (LBM.cu:1427-1503)
-void save_tecplot(FILE* f, double t, ehd_box* b, const ehd_params& p, const ehd_derived& d, bool first)
+void io(int rc, const char* what)
 {
-    Fields F(p);
-    F.read(b);
-    for (auto* v : {&F.rho, &F.q, &F.ux, &F.uy, &F.uz}) F.extrapolate(*v, true, true);
-    if (first) std::fprintf(f, "%s\n", "VARIABLES=\"x\",\"y\",\"z\",\"u\",\"v\",\"w\",\"p\",\"charge\",\"phi\",\"Ex\",\"Ey\",\"Ez\"");
-    std::fprintf(f, "\n");
-    std::fprintf(f, "ZONE T=\"t=%g\", F=POINT, I = %d, J = %d, K = %d\n", t, p.nx, p.ny, p.nz);
-    for (int z = 0; z < p.nz; ++z)
-        for (int y = 0; y < p.ny; ++y)
-            for (int x = 0; x < p.nx; ++x) {
-                const size_t i = F.idx(x, y, z);
-                std::fprintf(f, "%g %g %g %g %g %g %g %g %10.6f %10.6f %10.6f %10.6f\n", d.dx * x, d.dy * y, d.dz * z,
-                             F.ux[i], F.uy[i], F.uz[i], F.rho[i], F.q[i], F.phi[i], F.ex[i], F.ey[i], F.ez[i]);
-            }
-}
-
-// save_data_end (LBM.cu:1505-1566)
-void save_end(FILE* f, double t, ehd_box* b, const ehd_params& p)
-{
-    Fields F(p);
-    F.read(b);
-    for (auto* v : {&F.rho, &F.q, &F.ux, &F.uy, &F.uz}) F.extrapolate(*v, true, true);
-    for (int z = 0; z < p.nz; ++z)
-        for (int y = 0; y < p.ny; ++y)
-            for (int x = 0; x < p.nx; ++x) {
-                const size_t i = F.idx(x, y, z);
-                std::fprintf(f, "%10.6f %10.6f %10.6f %10.6f %10.6f %10.6f %10.6f %10.6f %10.6f %10.6f\n", t, F.ux[i],
-                             F.uy[i], F.uz[i], F.rho[i], F.q[i], F.phi[i], F.ex[i], F.ey[i], F.ez[i]);
-            }
-}
-
-// save_data_dmd (LBM.cu:1568-1594)
-void save_dmd(FILE* f, double t, ehd_box* b, const ehd_params& p)
-{
-    Fields F(p);
-    F.read(b);
-    F.extrapolate(F.uz, true, true);
-    for (size_t i = 0; i < F.uz.size(); ++i) std::fprintf(f, "%10.6f %10.6f\n", t, F.uz[i]);
-}
-
-// record_umax (LBM.cu:1673-1713)
-void record_umax(FILE* f, double t, ehd_box* b, const ehd_params& p)
-{
-    Fields F(p);
-    F.read(b);
-    F.extrapolate(F.uz, false, true);
-    double umax = 0;
-    for (double v : F.uz) umax = (umax > v) ? umax : v;
-    std::fprintf(f, "%10.6f %10.6f\n", t, umax);
-}
-
-// read_data (LBM.cu:1596-1643): data_end.dat -> fields, returns time
-bool read_restart(const std::string& path, const ehd_params& p, std::vector<double>& f9, double& t)
-{
-    FILE* f = std::fopen(path.c_str(), "r");
-    if (!f) return false;
-    const size_t S = (size_t)p.nx * p.ny * p.nz;
-    f9.assign(9 * S, 0.0);
-    for (size_t i = 0; i < S; ++i) {
-        double v[10];
-        if (std::fscanf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
-                        &v[7], &v[8], &v[9]) != 10) {
-            std::fclose(f);
-            return false;
-        }
-        t = v[0];
-        // file columns: t, u, v, w, rho, charge, phi, Ex, Ey, Ez -> rho, q, phi, u, v, w, Ex, Ey, Ez
-        f9[0 * S + i] = v[4];
-        f9[1 * S + i] = v[5];
-        f9[2 * S + i] = v[6];
-        f9[3 * S + i] = v[1];
-        f9[4 * S + i] = v[2];
-        f9[5 * S + i] = v[3];
-        f9[6 * S + i] = v[7];
-        f9[7 * S + i] = v[8];
-        f9[8 * S + i] = v[9];
-    }
-    std::fclose(f);
-    return true;
+    if (rc != EHD_OK) die(rc, what);
 }
 
 }  // namespace
@@ -242,10 +152,8 @@ int main(int argc, char** argv)
     std::vector<double> f9;
     double t = 0.0;
     if (o.restart) {
-        if (!read_restart(o.outdir + "/data_end.dat", p, f9, t)) {
-            std::fprintf(stderr, "Error: cannot read %s/data_end.dat\n", o.outdir.c_str());
-            return 1;
-        }
+        f9.assign(9 * (size_t)p.nx * p.ny * p.nz, 0.0);
+        io(ehd_io_read_end((o.outdir + "/data_end.dat").c_str(), p.nx, p.ny, p.nz, f9.data(), &t), "read_data");
         p.init_mode = EHD_INIT_FIELDS;
     }
     CALL(ehd_derive(&p, &d));
@@ -272,11 +180,13 @@ int main(int argc, char** argv)
     CALL(ehd_create(&p, &b));
     CALL(ehd_init(b, o.restart ? f9.data() : nullptr));
 
-    FILE* fout = nullptr;
-    FILE* fumax = std::fopen((o.outdir + "/umax.dat").c_str(), "wb+");
+    const std::string f_data = o.outdir + "/data.dat", f_umax = o.outdir + "/umax.dat";
+    const std::string f_end = o.outdir + "/data_end.dat";
+    Fields F(p);
+    bool umax_open = false;
     if (!o.quiet_files) {
-        fout = std::fopen((o.outdir + "/data.dat").c_str(), "wb+");
-        if (fout) save_tecplot(fout, t, b, p, d, true);
+        F.read(b);
+        io(ehd_io_tecplot(f_data.c_str(), 0, 1, t, p.nx, p.ny, p.nz, d.dx, d.dy, d.dz, F.f9.data()), "data.dat");
     }
 
     const auto begin = std::chrono::steady_clock::now();
@@ -293,24 +203,23 @@ int main(int argc, char** argv)
         CALL(ehd_step(b, (int)n));
         t += d.dt * (double)n;
         i = j;
-        if (fout && i % o.nsave == 1) save_tecplot(fout, t, b, p, d, false);
-        if (i % o.ndmd == 1) {
+        const bool save = !o.quiet_files && i % o.nsave == 1, dmd = i % o.ndmd == 1;
+        const bool cur = i % o.print_current == 1;
+        if (save || (dmd && !o.quiet_files) || cur) F.read(b);
+        if (save) io(ehd_io_tecplot(f_data.c_str(), 1, 0, t, p.nx, p.ny, p.nz, d.dx, d.dy, d.dz, F.f9.data()),
+                     "data.dat");
+        if (dmd) {
             std::printf("Iteration: %lld, physical time: %g.\n", i, t);
             if (!o.quiet_files) {
                 char name[128];
                 std::snprintf(name, sizeof(name), "%g.dat", t);
-                FILE* f2 = std::fopen((o.outdir + "/" + name).c_str(), "wb+");
-                if (f2) {
-                    save_dmd(f2, t, b, p);
-                    std::fclose(f2);
-                }
+                io(ehd_io_dmd((o.outdir + "/" + name).c_str(), t, p.nx, p.ny, p.nz, F.at(5)), name);
             }
         }
-        if (i % o.print_current == 1) {
-            ehd_diag dg;
-            CALL(ehd_diagnostics(b, &dg));
-            std::printf("Current = %g\n", dg.current);
-            if (fumax) record_umax(fumax, t, b, p);
+        if (cur) {
+            std::printf("Current = %g\n", ehd_io_current(p.nx, p.ny, p.nz, d.K, d.dy, F.at(1), F.at(8)));
+            io(ehd_io_umax(f_umax.c_str(), umax_open ? 1 : 0, t, p.nx, p.ny, p.nz, F.at(5)), "umax.dat");
+            umax_open = true;
         }
         ++i;
     }
@@ -323,20 +232,11 @@ int main(int argc, char** argv)
     std::printf("           clock runtime: %.3f (s)\n", runtime);
     std::printf("                   speed: %.2f (Mlups)\n", nodes / (1e6 * runtime));
 
-    if (fout) {
-        save_tecplot(fout, t, b, p, d, false);
-        std::fclose(fout);
-    }
-    if (fumax) {
-        record_umax(fumax, t, b, p);
-        std::fclose(fumax);
-    }
+    F.read(b);
+    io(ehd_io_umax(f_umax.c_str(), umax_open ? 1 : 0, t, p.nx, p.ny, p.nz, F.at(5)), "umax.dat");
     if (!o.quiet_files) {
-        FILE* fend = std::fopen((o.outdir + "/data_end.dat").c_str(), "wb+");
-        if (fend) {
-            save_end(fend, t, b, p);
-            std::fclose(fend);
-        }
+        io(ehd_io_tecplot(f_data.c_str(), 1, 0, t, p.nx, p.ny, p.nz, d.dx, d.dy, d.dz, F.f9.data()), "data.dat");
+        io(ehd_io_data_end(f_end.c_str(), t, p.nx, p.ny, p.nz, F.f9.data()), "data_end.dat");
     }
     ehd_destroy(b);
     return 0;

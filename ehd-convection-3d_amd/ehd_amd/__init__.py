@@ -68,6 +68,8 @@ EXPORTS = (
     "ehd_write_charge", "ehd_read_populations", "ehd_write_populations", "ehd_diagnostics",
     "ehd_set_nusselt_reference", "ehd_profile_enable", "ehd_profile_read", "ehd_profile_reset",
     "ehd_sync", "ehd_last_error", "ehd_abi_version", "ehd_save_state", "ehd_load_state",
+    # host-only file layer, include/ehd_io.h
+    "ehd_io_tecplot", "ehd_io_data_end", "ehd_io_read_end", "ehd_io_dmd", "ehd_io_umax", "ehd_io_current",
 )
 
 _lib = None
@@ -106,13 +108,21 @@ def lib() -> C.CDLL:
     L.ehd_sync.argtypes = [vp]
     L.ehd_save_state.argtypes = [vp, C.c_char_p]
     L.ehd_load_state.argtypes = [vp, C.c_char_p]
+    cs, i_ = C.c_char_p, C.c_int
+    L.ehd_io_tecplot.argtypes = [cs, i_, i_, C.c_double, i_, i_, i_, C.c_double, C.c_double, C.c_double, dp]
+    L.ehd_io_data_end.argtypes = [cs, C.c_double, i_, i_, i_, dp]
+    L.ehd_io_read_end.argtypes = [cs, i_, i_, i_, dp, dp]
+    L.ehd_io_dmd.argtypes = [cs, C.c_double, i_, i_, i_, dp]
+    L.ehd_io_umax.argtypes = [cs, i_, C.c_double, i_, i_, i_, dp]
+    L.ehd_io_current.argtypes = [i_, i_, i_, C.c_double, C.c_double, dp, dp]
     L.ehd_last_error.argtypes = []
     L.ehd_last_error.restype = C.c_char_p
     L.ehd_abi_version.argtypes = []
     for name in EXPORTS:
         f = getattr(L, name)
-        if name not in ("ehd_destroy", "ehd_last_error"):
+        if name not in ("ehd_destroy", "ehd_last_error", "ehd_io_current"):
             f.restype = C.c_int
+    L.ehd_io_current.restype = C.c_double
     _lib = L
     return L
 
@@ -259,3 +269,51 @@ class Box:
 
     def load_state(self, path: str):
         _check(lib().ehd_load_state(self._h, os.fsencode(path)))
+
+
+# ---- host-only file layer (include/ehd_io.h): the driver's output files ----
+
+def _f9(fields9, S):
+    a = np.ascontiguousarray(fields9, dtype=np.float64)
+    if a.size != 9 * S:
+        raise ValueError("fields9 must hold 9*S doubles")
+    return a
+
+
+def write_tecplot(path, t, fields9, shape, spacing, first=False, append=False):
+    nz, ny, nx = shape
+    a = _f9(fields9, nx * ny * nz)
+    _check(lib().ehd_io_tecplot(os.fsencode(path), int(append), int(first), float(t), nx, ny, nz,
+                                *map(float, spacing), _dp(a)))
+
+
+def write_data_end(path, t, fields9, shape):
+    nz, ny, nx = shape
+    a = _f9(fields9, nx * ny * nz)
+    _check(lib().ehd_io_data_end(os.fsencode(path), float(t), nx, ny, nz, _dp(a)))
+
+
+def read_data_end(path, shape):
+    nz, ny, nx = shape
+    out = np.empty((9, nz, ny, nx))
+    t = C.c_double()
+    _check(lib().ehd_io_read_end(os.fsencode(path), nx, ny, nz, _dp(out), C.byref(t)))
+    return out, t.value
+
+
+def write_dmd(path, t, uz, shape):
+    nz, ny, nx = shape
+    a = np.ascontiguousarray(uz, dtype=np.float64)
+    _check(lib().ehd_io_dmd(os.fsencode(path), float(t), nx, ny, nz, _dp(a)))
+
+
+def write_umax(path, t, uz, shape, append=False):
+    nz, ny, nx = shape
+    a = np.ascontiguousarray(uz, dtype=np.float64)
+    _check(lib().ehd_io_umax(os.fsencode(path), int(append), float(t), nx, ny, nz, _dp(a)))
+
+
+def host_current(q, ez, K, dy):
+    nz, ny, nx = q.shape
+    a, b = (np.ascontiguousarray(v, dtype=np.float64) for v in (q, ez))
+    return lib().ehd_io_current(nx, ny, nz, float(K), float(dy), _dp(a), _dp(b))

@@ -9,13 +9,15 @@ import pytest
 import ehd_amd as E
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "ehd.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("ehd.h", "ehd_io.h")]
 
 
 def declared_functions():
-    txt = open(HEADER).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(ehd_[a-z_0-9]+)\s*\(", txt)))
+    names = set()
+    for h in HEADERS:
+        txt = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(ehd_[a-z_0-9]+)\s*\(", txt))
+    return sorted(names)
 
 
 def test_header_declares_the_binding_exports():
