@@ -3,6 +3,7 @@
 // diagnostics.  Host C++ over the HIP runtime; one HIP stream per box.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -272,6 +273,83 @@ double hydrostatic_e0(double C)
 
 }  // namespace
 
+namespace {
+// Analytic unipolar-injection profile sampled on the grid: q(z), phi(z) and
+// rho(z) in one table of 3*nz doubles.  Closed form of the 1-D hydrostatic
+// state: E(zeta)^2 = E0^2 + 2 C E0 zeta (nondimensional), q = C E0/E, phi from
+// integrating E, with E0 fixed by phi(1) = 0.
+std::vector<double> hydrostatic_table(const ehd_params& p, const ehd_derived& d)
+{
+    const int nz = p.nz;
+    std::vector<double> tab(3 * (size_t)nz);
+    const double C = d.charge0 * p.lz * p.lz / (p.voltage * p.eps);
+    const double e0 = hydrostatic_e0(C);
+    const double qs = p.eps * p.voltage / (p.lz * p.lz);
+    // uniform current density J = q Ez balances the pressure: dp/dz = J
+    const double J = C * e0 * p.eps * p.voltage * p.voltage / (p.lz * p.lz * p.lz);
+    for (int z = 0; z < nz; ++z) {
+        const double zeta = d.dz * z / p.lz;
+        const double e = std::sqrt(e0 * e0 + 2.0 * C * e0 * zeta);
+        tab[z] = C * e0 / e * qs;
+        double ph = p.voltage * (1.0 - (e * e * e - e0 * e0 * e0) / (3.0 * C * e0));
+        if (z == 0) ph = p.voltage;
+        if (z == nz - 1) ph = 0.0;
+        tab[nz + z] = ph;
+        tab[2 * nz + z] = p.hydro_density ? p.rho0 + J * (d.dz * z - 0.5 * p.lz) / d.cs_square : p.rho0;
+    }
+    return tab;
+}
+
+// current() of LBM.cu:1646-1671 on the sampled hydrostatic profile: q
+// extrapolated to the top wall, Ez of the wall copy rule (poisson.cu:62-73).
+double hydrostatic_current(const ehd_params& p, const ehd_derived& d)
+{
+    const std::vector<double> tab = hydrostatic_table(p, d);
+    const int nz = p.nz;
+    const double qt = 2.0 * tab[nz - 2] - tab[nz - 3];
+    const double ez = 0.5 * (tab[nz + nz - 3] - tab[nz + nz - 1]) / d.dz;
+    return (double)p.nx * (double)p.ny * (qt * ez) * d.K * d.dy * d.dy;
+}
+}  // namespace
+
+namespace {
+struct CkptHeader {
+    char magic[8];
+    int abi, precision, nx, ny, nz, direct, ebuf, pad;
+    long long step;
+    double t;
+    double T, C, M, Fe, lambda_f, lambda_c;
+};
+
+int ckpt_io(ehd_box* b, FILE* f, bool save)
+{
+    const size_t S = b->S, es = b->esz;
+    const int A = b->cur, B = 1 - b->cur;
+    struct Item {
+        void* dev;
+        size_t bytes;
+    } items[] = {
+        {b->f[A], 26 * S * es}, {b->f0[A], S * es}, {b->h[A], 26 * S * es}, {b->h0[A], S * es},
+        {b->f0[B], S * es},    // frozen wall rest populations live in both buffers
+        {b->rho, S * es}, {b->ux, S * es}, {b->uy, S * es}, {b->uz, S * es}, {b->q, S * es}, {b->phi, S * es},
+        {b->phibar, (size_t)b->nz * sizeof(double)},
+        {b->ex, S * sizeof(double)}, {b->ey, S * sizeof(double)}, {b->ez, S * sizeof(double)},
+    };
+    std::vector<unsigned char> buf;
+    for (auto& it : items) {
+        buf.resize(it.bytes);
+        if (save) {
+            HIPCHK(hipMemcpy(buf.data(), it.dev, it.bytes, hipMemcpyDeviceToHost));
+            if (std::fwrite(buf.data(), 1, it.bytes, f) != it.bytes) return set_error(EHD_EINVAL, "checkpoint write failed");
+        } else {
+            if (std::fread(buf.data(), 1, it.bytes, f) != it.bytes) return set_error(EHD_EINVAL, "checkpoint truncated");
+            HIPCHK(hipMemcpy(it.dev, buf.data(), it.bytes, hipMemcpyHostToDevice));
+        }
+    }
+    return EHD_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int ehd_abi_version(void) { return EHD_ABI_VERSION; }
@@ -442,6 +520,18 @@ void ehd_destroy(ehd_box* b)
     delete b;
 }
 
+
+int ehd_hydrostatic_current(const ehd_params* p, double* current)
+{
+    if (!p || !current) return set_error(EHD_EINVAL, "null argument");
+    int rc = validate(p);
+    if (rc) return rc;
+    ehd_derived d;
+    if ((rc = ehd_derive(p, &d))) return rc;
+    *current = hydrostatic_current(*p, d);
+    return EHD_OK;
+}
+
 int ehd_init(ehd_box* b, const double* fields9)
 {
     if (!b) return set_error(EHD_EINVAL, "null box");
@@ -466,22 +556,7 @@ int ehd_init(ehd_box* b, const double* fields9)
         HIPCHK(hipMemsetAsync(b->ez, 0, S * sizeof(double), b->st));
     } else if (p.init_mode == EHD_INIT_HYDROSTATIC) {
         const int nz = b->nz;
-        std::vector<double> tab(3 * (size_t)nz);
-        const double C = b->der.charge0 * p.lz * p.lz / (p.voltage * p.eps);
-        const double e0 = hydrostatic_e0(C);
-        const double qs = p.eps * p.voltage / (p.lz * p.lz);
-        // uniform current density J = q Ez balances the pressure: dp/dz = J
-        const double J = C * e0 * p.eps * p.voltage * p.voltage / (p.lz * p.lz * p.lz);
-        for (int z = 0; z < nz; ++z) {
-            const double zeta = b->der.dz * z / p.lz;
-            const double e = std::sqrt(e0 * e0 + 2.0 * C * e0 * zeta);
-            tab[z] = C * e0 / e * qs;
-            double ph = p.voltage * (1.0 - (e * e * e - e0 * e0 * e0) / (3.0 * C * e0));
-            if (z == 0) ph = p.voltage;
-            if (z == nz - 1) ph = 0.0;
-            tab[nz + z] = ph;
-            tab[2 * nz + z] = p.hydro_density ? p.rho0 + J * (b->der.dz * z - 0.5 * p.lz) / b->der.cs_square : p.rho0;
-        }
+        const std::vector<double> tab = hydrostatic_table(p, b->der);
         rc = alloc((void**)&dqz, tab.size() * sizeof(double));
         if (rc) return rc;
         hipError_t e1 = hipMemcpyAsync(dqz, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, b->st);
@@ -505,6 +580,7 @@ int ehd_init(ehd_box* b, const double* fields9)
         HIPCHK(hipMemcpyAsync(b->ez, fields9 + 8 * S, S * 8, hipMemcpyHostToDevice, b->st));
     }
     b->cur = 0;
+    b->current_hs = hydrostatic_current(p, b->der);
     HIPCHK(launch_init_equilibrium(b->prec, b->lp, rho, q, ux, uy, uz, b->ex, b->ey, b->ez, b->f[0], b->f0[0], b->h[0],
                                    b->h0[0], b->st));
     // the frozen wall rest populations must be present in both buffers
@@ -690,43 +766,6 @@ int ehd_write_populations(ehd_box* b, const double* f0, const double* f1, const 
     return EHD_OK;
 }
 
-namespace {
-struct CkptHeader {
-    char magic[8];
-    int abi, precision, nx, ny, nz, direct, ebuf, pad;
-    long long step;
-    double t;
-    double T, C, M, Fe, lambda_f, lambda_c;
-};
-
-int ckpt_io(ehd_box* b, FILE* f, bool save)
-{
-    const size_t S = b->S, es = b->esz;
-    const int A = b->cur, B = 1 - b->cur;
-    struct Item {
-        void* dev;
-        size_t bytes;
-    } items[] = {
-        {b->f[A], 26 * S * es}, {b->f0[A], S * es}, {b->h[A], 26 * S * es}, {b->h0[A], S * es},
-        {b->f0[B], S * es},    // frozen wall rest populations live in both buffers
-        {b->rho, S * es}, {b->ux, S * es}, {b->uy, S * es}, {b->uz, S * es}, {b->q, S * es}, {b->phi, S * es},
-        {b->phibar, (size_t)b->nz * sizeof(double)},
-        {b->ex, S * sizeof(double)}, {b->ey, S * sizeof(double)}, {b->ez, S * sizeof(double)},
-    };
-    std::vector<unsigned char> buf;
-    for (auto& it : items) {
-        buf.resize(it.bytes);
-        if (save) {
-            HIPCHK(hipMemcpy(buf.data(), it.dev, it.bytes, hipMemcpyDeviceToHost));
-            if (std::fwrite(buf.data(), 1, it.bytes, f) != it.bytes) return set_error(EHD_EINVAL, "checkpoint write failed");
-        } else {
-            if (std::fread(buf.data(), 1, it.bytes, f) != it.bytes) return set_error(EHD_EINVAL, "checkpoint truncated");
-            HIPCHK(hipMemcpy(it.dev, buf.data(), it.bytes, hipMemcpyHostToDevice));
-        }
-    }
-    return EHD_OK;
-}
-}  // namespace
 
 int ehd_save_state(ehd_box* b, const char* path)
 {
@@ -802,6 +841,35 @@ int ehd_diagnostics(ehd_box* b, ehd_diag* out)
     out->step = b->step;
     out->time = b->t;
     if (r[4] != 0.0) return set_error(EHD_ENONFINITE, "non-finite charge or velocity at %.0f nodes (step %lld)", r[4], b->step);
+    return EHD_OK;
+}
+
+int ehd_run_to_steady(ehd_box* b, int check_every, double tol, long long max_steps, ehd_steady* out)
+{
+    if (!b || !out) return set_error(EHD_EINVAL, "null argument");
+    if (check_every < 1 || !(tol >= 0.0) || max_steps < 0) return set_error(EHD_EINVAL, "bad steady-state criteria");
+    if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
+    ehd_diag dg;
+    int rc = ehd_diagnostics(b, &dg);
+    if (rc) return rc;
+    double prev = dg.nusselt;
+    long long taken = 0;
+    out->converged = 0;
+    while (taken < max_steps) {
+        const int n = (int)std::min<long long>(check_every, max_steps - taken);
+        if ((rc = ehd_step(b, n))) return rc;
+        taken += n;
+        if ((rc = ehd_diagnostics(b, &dg))) return rc;
+        out->delta = std::fabs(dg.nusselt - prev);
+        prev = dg.nusselt;
+        if (n == check_every && out->delta < tol) {
+            out->converged = 1;
+            break;
+        }
+    }
+    out->steps = taken;
+    out->nusselt = dg.nusselt;
+    out->current = dg.current;
     return EHD_OK;
 }
 

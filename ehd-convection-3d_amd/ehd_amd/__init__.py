@@ -62,12 +62,18 @@ class Profile(C.Structure):
                 ("step_ms", C.c_double), ("steps", C.c_longlong)]
 
 
+class Steady(C.Structure):
+    _fields_ = [("steps", C.c_longlong), ("converged", C.c_int), ("nusselt", C.c_double),
+                ("current", C.c_double), ("delta", C.c_double)]
+
+
 EXPORTS = (
     "ehd_params_default", "ehd_derive", "ehd_compute_parameters", "ehd_create", "ehd_destroy",
     "ehd_init", "ehd_step", "ehd_lbm_step", "ehd_poisson", "ehd_read_fields", "ehd_write_phi",
     "ehd_write_charge", "ehd_read_populations", "ehd_write_populations", "ehd_diagnostics",
     "ehd_set_nusselt_reference", "ehd_profile_enable", "ehd_profile_read", "ehd_profile_reset",
     "ehd_sync", "ehd_last_error", "ehd_abi_version", "ehd_save_state", "ehd_load_state",
+    "ehd_hydrostatic_current", "ehd_run_to_steady",
     # host-only file layer, include/ehd_io.h
     "ehd_io_tecplot", "ehd_io_data_end", "ehd_io_read_end", "ehd_io_dmd", "ehd_io_umax", "ehd_io_current",
 )
@@ -106,6 +112,8 @@ def lib() -> C.CDLL:
     L.ehd_profile_read.argtypes = [vp, C.POINTER(Profile)]
     L.ehd_profile_reset.argtypes = [vp]
     L.ehd_sync.argtypes = [vp]
+    L.ehd_hydrostatic_current.argtypes = [C.POINTER(Params), dp]
+    L.ehd_run_to_steady.argtypes = [vp, C.c_int, C.c_double, C.c_longlong, C.POINTER(Steady)]
     L.ehd_save_state.argtypes = [vp, C.c_char_p]
     L.ehd_load_state.argtypes = [vp, C.c_char_p]
     cs, i_ = C.c_char_p, C.c_int
@@ -158,6 +166,12 @@ def make_params(nx, ny, nz, T=200.0, C_=10.0, M=10.0, Fe=4000.0, Pr=10.0, lam=0.
     p.hydro_density = hydro_density
     p.perturb_pattern = pattern
     return p
+
+
+def hydrostatic_current(p: Params) -> float:
+    v = C.c_double()
+    _check(lib().ehd_hydrostatic_current(C.byref(p), C.byref(v)))
+    return v.value
 
 
 def derive(p: Params) -> Derived:
@@ -263,6 +277,11 @@ class Box:
 
     def sync(self):
         _check(lib().ehd_sync(self._h))
+
+    def run_to_steady(self, check_every: int = 1000, tol: float = 1e-6, max_steps: int = 10**6) -> Steady:
+        s = Steady()
+        _check(lib().ehd_run_to_steady(self._h, int(check_every), float(tol), int(max_steps), C.byref(s)))
+        return s
 
     def save_state(self, path: str):
         _check(lib().ehd_save_state(self._h, os.fsencode(path)))

@@ -157,8 +157,25 @@ int ehd_load_state(ehd_box* b, const char* path);
 /* FP64 reductions on the device (current, total charge, umax, ...). */
 int ehd_diagnostics(ehd_box* b, ehd_diag* out);
 
-/* Reference current of the hydrostatic state, used for the Nusselt number. */
+/* Reference current of the Nusselt number Nu = I / I_hs.  ehd_init sets
+ * I_hs to ehd_hydrostatic_current of the box's parameters, so a box started
+ * from the hydrostatic state has Nu = 1 at step 0; this call overrides it
+ * (e.g. with the current of a converged 1-D run). */
 int ehd_set_nusselt_reference(ehd_box* b, double current_hs);
+
+/* Collector current of the analytic hydrostatic state sampled on the grid,
+ * evaluated as current() (LBM.cu:1646-1671) evaluates it.  Host only. */
+int ehd_hydrostatic_current(const ehd_params* p, double* current);
+
+/* Steady-state detector: steps the box in chunks of check_every steps and
+ * stops once |Nu(after chunk) - Nu(before chunk)| < tol (e.g. 1e-6 over 1000
+ * steps), or after max_steps.  steps counts the steps taken by this call. */
+typedef struct ehd_steady {
+    long long steps;
+    int converged;
+    double nusselt, current, delta;
+} ehd_steady;
+int ehd_run_to_steady(ehd_box* b, int check_every, double tol, long long max_steps, ehd_steady* out);
 
 /* Per-kernel timing with HIP events on the box's stream.  When enabled,
  * every launch of the fused LBM kernel and of the Poisson chain is bracketed

@@ -75,3 +75,17 @@ def test_invalid_params_rejected_on_host(bad):
     assert ei.value.code == -1
     assert E.lib().ehd_last_error()
     del kw
+
+
+@pytest.mark.parametrize("C", [5.0, 10.0, 20.0])
+def test_hydrostatic_current_matches_oracle(C):
+    """ehd_hydrostatic_current = the oracle's current() (LBM.cu:1646-1671) of
+    the hydrostatic initial state, so Nu = 1 there by construction."""
+    from oracle import pyoracle as O
+    nx, ny, nz = 12, 10, 17
+    ob = O.Box(O.make_params(nx, ny, nz, T=200.0, M=10.0, C_=C, Fe=4000.0, lam=0.25))
+    ob.init_hydrostatic(1, 0.0, 0)
+    ref = ob.current()
+    got = E.hydrostatic_current(E.make_params(nx, ny, nz, T=200.0, C_=C, lam=0.25))
+    assert ref > 0
+    assert abs(got - ref) <= 1e-13 * ref, (got, ref)

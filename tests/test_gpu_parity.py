@@ -250,6 +250,25 @@ def test_diagnostics_current_vs_oracle():
         assert d.step == 3
 
 
+def test_nusselt_is_one_at_hydrostatic_start_and_steady_detector():
+    """Nu = I / I_hs is 1 on the hydrostatic start; below the instability
+    threshold the run relaxes to a steady state that the detector reports,
+    with Nu close to 1 (the discrete LBM equilibrium differs from the sampled
+    analytic profile only by discretisation error)."""
+    p = E.make_params(16, 16, 17, T=50.0, C_=10.0, lam=0.25, precision=E.FP64, init_mode=E.INIT_HYDROSTATIC,
+                      seed=4, amp=1e-4)
+    with E.Box(p) as b:
+        b.init()
+        d0 = b.diagnostics()
+        assert abs(d0.nusselt - 1.0) < 1e-12, d0.nusselt
+        s = b.run_to_steady(check_every=500, tol=1e-7, max_steps=40000)
+        print(f"steady after {s.steps} steps: Nu {s.nusselt:.9f} delta {s.delta:.2e}")
+        assert s.converged == 1 and s.delta < 1e-7
+        assert abs(s.nusselt - 1.0) < 2e-2
+        d1 = b.diagnostics()
+        assert d1.step == s.steps and d1.nusselt == s.nusselt
+
+
 def _coupled_errors(ga, ob):
     """rel-L2 of u, q (interior) and phi of a field dict or oracle box against the oracle box ob."""
     def get(src, key, name):
