@@ -43,8 +43,9 @@ def parse():
     ap.add_argument("--ny", type=int, default=256)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--precision", choices=("fp32s", "fp64"), default="fp32s")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps in the CPU sample (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle OpenMP threads (0: all host cores)")
+    ap.add_argument("--cpu-steps", type=int, default=8, help="oracle steps in the CPU sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle OpenMP threads (0: OMP_NUM_THREADS, else the cores this process may use)")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch events (value only)")
     return ap.parse_args()
 
@@ -63,7 +64,7 @@ def traffic_from_profiles(workload: str):
 def cpu_baseline(args, workload):
     from oracle import pyoracle as O
 
-    threads = args.cpu_threads or (os.cpu_count() or 1)
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     O.set_threads(threads)
     p = O.make_params(args.nx, args.ny, args.nz, T=400.0, M=10.0, C_=10.0, Fe=4000.0, lam=0.25)
     b = O.Box(p)

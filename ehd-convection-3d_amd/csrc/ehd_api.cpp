@@ -72,6 +72,8 @@ struct ehd_box {
     double* shift_rs = nullptr;   // FP32S storage shifts (device)
     double* shift_qs = nullptr;
     float* shift_f = nullptr;
+    PoissonJob* pjob = nullptr;   // the box's one-entry Poisson job table (device)
+    int tune_lpb = 0, tune_ylines = 0;   // Poisson lines per block (0: largest that fits)
     long long step = 0;
     double t = 0.0;
     double current_hs = 0.0;
@@ -191,7 +193,7 @@ void free_box(ehd_box* b)
     void* ptrs[] = {b->f[0], b->f[1], b->f0[0], b->f0[1], b->h[0], b->h[1], b->h0[0], b->h0[1],
                     b->rho, b->ux, b->uy, b->uz, b->q, b->phi, b->phibar, b->ex, b->ey, b->ez,
                     b->spec, b->twx, b->twy, b->diag_scr, b->diag_out,
-                    b->shift_rs, b->shift_qs, b->shift_f};
+                    b->shift_rs, b->shift_qs, b->shift_f, b->pjob};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& pr : b->lbm_pairs) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
@@ -234,6 +236,23 @@ int run_lbm(ehd_box* b, bool macro)
     return EHD_OK;
 }
 
+PoissonGroup poisson_group_of(const ehd_box* b)
+{
+    PoissonGroup g{};
+    g.nx = b->nx;
+    g.ny = b->ny;
+    g.max_n = b->nz - 2;
+    g.njobs = 1;
+    g.split_mean = b->prec == Prec::F32S ? 1 : 0;
+    g.lpb = b->tune_lpb;
+    g.ylines = b->tune_ylines;
+    g.zs_partition = poisson_shape(b->prec, b->nx, b->ny, b->nz).zs_partition;
+    g.twx = b->twx;
+    g.twy = b->twy;
+    g.jobs = b->pjob;
+    return g;
+}
+
 int run_poisson(ehd_box* b)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -243,8 +262,7 @@ int run_poisson(ehd_box* b)
         if (!e0 || !e1) return set_error(EHD_EHIP, "hipEventCreate failed");
         HIPCHK(hipEventRecord(e0, b->st));
     }
-    PoissonBufs pb{b->nx, b->ny, b->nz, b->nxh, b->spec, b->twx, b->twy, b->phibar};
-    hipError_t e = launch_poisson(b->prec, b->lp, pb, b->q, b->phi, b->prm.voltage, b->prm.eps, b->st);
+    hipError_t e = launch_poisson_group(b->prec, poisson_group_of(b), b->st);
     if (e != hipSuccess) return set_error(EHD_EHIP, "Poisson launch failed: %s", hipGetErrorString(e));
     if (b->prof) {
         HIPCHK(hipEventRecord(e1, b->st));
@@ -507,6 +525,17 @@ int ehd_create(const ehd_params* p, ehd_box** out)
         if (hipMemcpy(b->twx, tx.data(), tx.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(b->twy, ty.data(), ty.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
             return bail(set_error(EHD_EHIP, "twiddle upload failed"));
+    }
+    {
+        const PoissonShape ps = poisson_shape(b->prec, b->nx, b->ny, b->nz);
+        if (ps.max_x_lines == 0 || ps.max_y_lines == 0 || ps.zs_partition == 0)
+            return bail(set_error(EHD_EINVAL, "grid %dx%dx%d exceeds the Poisson solver's LDS tiles", b->nx, b->ny,
+                                  b->nz));
+        PoissonBufs pb{b->nx, b->ny, b->nz, b->nxh, b->spec, b->twx, b->twy, b->phibar};
+        const PoissonJob job = make_poisson_job(b->lp, pb, b->q, b->phi, p->voltage, p->eps);
+        if ((rc = alloc((void**)&b->pjob, sizeof(PoissonJob)))) return bail(rc);
+        if (hipMemcpy(b->pjob, &job, sizeof(job), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(set_error(EHD_EHIP, "Poisson job upload failed"));
     }
     *out = b;
     return EHD_OK;
@@ -841,6 +870,45 @@ int ehd_diagnostics(ehd_box* b, ehd_diag* out)
     out->step = b->step;
     out->time = b->t;
     if (r[4] != 0.0) return set_error(EHD_ENONFINITE, "non-finite charge or velocity at %.0f nodes (step %lld)", r[4], b->step);
+    return EHD_OK;
+}
+
+int ehd_hydrostatic_reference(ehd_box* b, int check_every, double tol, long long max_steps, double* current)
+{
+    if (!b || !current) return set_error(EHD_EINVAL, "null argument");
+    if (check_every < 1 || !(tol >= 0.0) || max_steps < 1) return set_error(EHD_EINVAL, "bad convergence criteria");
+    ehd_params p = b->prm;
+    p.nx = 2;
+    p.ny = 2;
+    p.precision = EHD_FP64;
+    p.init_mode = EHD_INIT_HYDROSTATIC;
+    p.perturb_amp = 0.0;
+    ehd_box* c = nullptr;
+    int rc = ehd_create(&p, &c);
+    if (rc) return rc;
+    struct Destroy {
+        ehd_box* c;
+        ~Destroy() { ehd_destroy(c); }
+    } guard{c};
+    if ((rc = ehd_init(c, nullptr))) return rc;
+    ehd_diag dg;
+    if ((rc = ehd_diagnostics(c, &dg))) return rc;
+    double prev = dg.current;
+    long long taken = 0;
+    bool converged = false;
+    while (taken < max_steps && !converged) {
+        const int n = (int)std::min<long long>(check_every, max_steps - taken);
+        if ((rc = ehd_step(c, n))) return rc;
+        taken += n;
+        if ((rc = ehd_diagnostics(c, &dg))) return rc;
+        converged = std::fabs(dg.current - prev) <= tol * std::fabs(dg.current);
+        prev = dg.current;
+    }
+    if (!converged)
+        return set_error(EHD_ESTATE, "hydrostatic reference not converged in %lld steps", max_steps);
+    // a laterally uniform state: the current scales with the number of columns
+    *current = dg.current * ((double)b->nx * (double)b->ny / 4.0);
+    b->current_hs = *current;
     return EHD_OK;
 }
 

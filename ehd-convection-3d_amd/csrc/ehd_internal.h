@@ -57,6 +57,15 @@ hipError_t launch_lbm_f64(const LbmParams& P, const PopBufs& pb, const MacroBufs
 hipError_t launch_lbm_f32s(const LbmParams& P, const PopBufs& pb, const MacroBufs& mb,
                            bool first, bool ebuf, bool macro, hipStream_t st);
 
+// Batched FP32S LBM: an opaque item per box (lbm_f32s_batch_item writes
+// lbm_f32s_batch_item_size() bytes), items in a device array sorted by tile0,
+// tile0 = number of 256-node tiles of the boxes before it.  grid = 0 launches
+// one workgroup per tile; a smaller grid makes the workgroups persistent.
+size_t lbm_f32s_batch_item_size();
+void lbm_f32s_batch_item(void* out, const LbmParams& P, const PopBufs& pb, const MacroBufs& mb, unsigned tile0);
+hipError_t launch_lbm_f32s_batch(const void* items, int nitems, unsigned total_tiles, unsigned grid, bool first,
+                                 bool ebuf, bool macro, hipStream_t st);
+
 // init_equilibrium (LBM.cu:98-276) from double macroscopic fields into the
 // pre-collision ("direct") representation of the path.
 hipError_t launch_init_equilibrium(Prec prec, const LbmParams& P, const double* rho, const double* q,
@@ -93,8 +102,32 @@ hipError_t launch_init_fields(const LbmParams& P, int mode, const double* qz, co
 int diag_blocks();
 
 // --- Poisson (poisson.hip) -------------------------------------------------
-hipError_t launch_poisson(Prec prec, const LbmParams& P, const PoissonBufs& pb, const void* q,
-                          void* phi, double voltage, double eps, hipStream_t st);
+// One box's share of a Poisson solve.  A batch runs many jobs of one (nx, ny)
+// in the same launches (job index = blockIdx.y); a single box is a group of one.
+struct PoissonJob {
+    const void* q;               // charge (path type), [nz][ny][nx]
+    void* spec;                  // spectrum workspace of the box
+    void* phi;                   // potential (path type)
+    double* phibar;              // FP32S plane means, nullptr for F64
+    int n;                       // interior planes nz-2
+    double kxs, kys, dz2, eps, voltage;
+};
+struct PoissonGroup {
+    int nx, ny, max_n, njobs;
+    int split_mean;              // FP32S
+    int lpb, ylines;             // lines per block of the x / y passes (0: largest that fits)
+    int zs_partition;            // threads per mode of the z solve (poisson_shape)
+    const void* twx;             // twiddles of nx and ny, path type
+    const void* twy;
+    const PoissonJob* jobs;      // device array of njobs
+};
+struct PoissonShape {
+    int max_x_lines, max_y_lines, zs_partition;
+};
+PoissonJob make_poisson_job(const LbmParams& P, const PoissonBufs& pb, const void* q, void* phi, double voltage,
+                            double eps);
+PoissonShape poisson_shape(Prec prec, int nx, int ny, int nz);
+hipError_t launch_poisson_group(Prec prec, const PoissonGroup& g, hipStream_t st);
 
 // --- diagnostics (diag.hip) --------------------------------------------------
 // out[0]=sum_xy q_top*Ez_top, out[1]=sum interior q, out[2]=max uz (top extrapolated),

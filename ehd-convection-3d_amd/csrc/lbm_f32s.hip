@@ -25,6 +25,8 @@
 // u + K E, no source, and the charge rates.  Arithmetic is FP32.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "ehd_internal.h"
 
 namespace ehd {
@@ -58,14 +60,11 @@ __device__ __forceinline__ float pulled(float v, int d, float w, float dup, floa
 __device__ __forceinline__ float ldnt(const float* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void stnt(float v, float* p) { __builtin_nontemporal_store(v, p); }
 
-// Blocks run in linear order (x fastest, then y, then z): measured faster on
-// MI355X than a remap that gives each XCD a contiguous slab of planes.
+// One node s of one box: pull, collide, boundary and injection.
 template <bool FIRST, bool EBUF>
-__global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c, PopBufs pb, MacroBufs mb,
-                                                       int macro)
+__device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Consts& c, const PopBufs& pb,
+                                              const MacroBufs& mb, int macro, unsigned s)
 {
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= P.S) return;
     const unsigned nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy, S = P.S;
     const unsigned x = s % nx, t = s / nx, y = t % ny, z = t / ny;
     const unsigned xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
@@ -278,10 +277,46 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c,
     }
 }
 
-}  // namespace
+// Blocks run in linear order (x fastest, then y, then z): measured faster on
+// MI355X than a remap that gives each XCD a contiguous slab of planes.
+template <bool FIRST, bool EBUF>
+__global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c, PopBufs pb, MacroBufs mb,
+                                                       int macro)
+{
+    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.S) return;
+    lbm_f32s_node<FIRST, EBUF>(P, c, pb, mb, macro, s);
+}
 
-hipError_t launch_lbm_f32s(const LbmParams& P, const PopBufs& pb, const MacroBufs& mb, bool first, bool ebuf,
-                           bool macro, hipStream_t st)
+// Many boxes of a batch in one persistent launch: the boxes' node ranges are
+// cut into 256-node tiles numbered across the batch (box i owns tiles
+// tile0[i] .. tile0[i+1]-1) and the workgroups stride over all tiles.
+struct BatchItem {
+    LbmParams P;
+    F32Consts c;
+    PopBufs pb;
+    MacroBufs mb;
+    unsigned tile0, ntiles;
+};
+
+template <bool FIRST, bool EBUF>
+__global__ void __launch_bounds__(256) lbm_f32s_batch_kernel(const BatchItem* __restrict__ items, int nitems,
+                                                             unsigned total_tiles, int macro)
+{
+    for (unsigned tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
+        int lo = 0, hi = nitems - 1;   // last item with tile0 <= tile
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].tile0 <= tile) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchItem& it = items[lo];
+        const unsigned s = (tile - it.tile0) * 256u + threadIdx.x;
+        if (s < it.P.S) lbm_f32s_node<FIRST, EBUF>(it.P, it.c, it.pb, it.mb, macro, s);
+    }
+}
+
+F32Consts make_f32consts(const LbmParams& P)
 {
     F32Consts c;
     const double cs2 = P.cs_square, cfl = P.cfl;
@@ -307,6 +342,43 @@ hipError_t launch_lbm_f32s(const LbmParams& P, const PopBufs& pb, const MacroBuf
     c.hdx = (float)(0.5 / P.dx);
     c.hdy = (float)(0.5 / P.dy);
     c.hdz = (float)(0.5 / P.dz);
+    return c;
+}
+
+}  // namespace
+
+size_t lbm_f32s_batch_item_size() { return sizeof(BatchItem); }
+
+void lbm_f32s_batch_item(void* out, const LbmParams& P, const PopBufs& pb, const MacroBufs& mb, unsigned tile0)
+{
+    BatchItem it{};
+    it.P = P;
+    it.c = make_f32consts(P);
+    it.pb = pb;
+    it.mb = mb;
+    it.tile0 = tile0;
+    it.ntiles = (P.S + 255) / 256;
+    std::memcpy(out, &it, sizeof(it));
+}
+
+hipError_t launch_lbm_f32s_batch(const void* items, int nitems, unsigned total_tiles, unsigned grid, bool first,
+                                 bool ebuf, bool macro, hipStream_t st)
+{
+    if (nitems < 1 || total_tiles == 0) return hipErrorInvalidValue;
+    const BatchItem* it = (const BatchItem*)items;
+    const unsigned g = (grid == 0 || grid > total_tiles) ? total_tiles : grid;
+    const int m = macro ? 1 : 0;
+    if (first && ebuf) lbm_f32s_batch_kernel<true, true><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
+    else if (first) lbm_f32s_batch_kernel<true, false><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
+    else if (ebuf) lbm_f32s_batch_kernel<false, true><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
+    else lbm_f32s_batch_kernel<false, false><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_lbm_f32s(const LbmParams& P, const PopBufs& pb, const MacroBufs& mb, bool first, bool ebuf,
+                           bool macro, hipStream_t st)
+{
+    const F32Consts c = make_f32consts(P);
     const unsigned nblk = (P.S + 255) / 256;
     dim3 g(nblk), b(256);
     int m = macro ? 1 : 0;

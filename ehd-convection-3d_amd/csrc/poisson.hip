@@ -18,7 +18,7 @@
 // mixed-radix FFTs in LDS (radix 4, 2, 3, 5 butterflies; any other prime
 // factor is done as a direct DFT stage).
 //
-// Spectral layout: [z-1][ky][kx], kx in [0, nx/2], pitch nxh = nx/2+1.
+// Spectral layout: [z-1][kx][ky], kx in [0, nx/2], ky fastest.
 // FP32S path: the spectra are float; mode (0,0) is solved in FP64 and kept
 // as the per-plane mean phibar[z] (FP64), and phi holds phi - phibar in
 // float, so that E = -grad(phi) loses no digits to the large wall potential.
@@ -162,13 +162,16 @@ __device__ cplx<T>* lds_fft(cplx<T>* A, cplx<T>* B, int n, int nlines, int ldl, 
 }
 
 // ---------------------------------------------------------------------------
-// Compile-time planned FFT (the grid sizes the configurations use): the radix
-// schedule, the strides and every index division are constants.
+// Compile-time planned FFT (the grid sizes the configurations use).  Radix 16
+// and 8 butterflies run in registers (as 4x4 and 4x2 with constant inner
+// twiddles), so a 256-point line takes two stages and one LDS exchange.  The
+// radix schedule, the strides and every index division are constants.
 constexpr int ct_radix(int n, int st)
 {
+    if (n <= 0) return 0;
     int r = n, s = 0;
-    const int pref[4] = {4, 2, 3, 5};
-    for (int i = 0; i < 4; ++i)
+    const int pref[6] = {16, 8, 4, 2, 3, 5};
+    for (int i = 0; i < 6; ++i)
         while (r % pref[i] == 0) {
             if (s == st) return pref[i];
             r /= pref[i];
@@ -182,6 +185,39 @@ constexpr int ct_nst(int n)
     while (ct_radix(n, s) != 0) ++s;
     return s;
 }
+// largest number of butterflies of one line over the stages (N / smallest radix)
+constexpr int ct_max_nb(int n)
+{
+    int m = 0;
+    for (int s = 0; s < ct_nst(n); ++s) m = (n / ct_radix(n, s) > m) ? n / ct_radix(n, s) : m;
+    return m;
+}
+// the plan covers n completely (no factor left for a runtime stage)
+constexpr bool ct_complete(int n)
+{
+    int r = n;
+    for (int s = 0; s < ct_nst(n); ++s) r /= ct_radix(n, s);
+    return r == 1;
+}
+
+// a * (c + i s)
+template <typename T>
+__device__ __forceinline__ cplx<T> crot(cplx<T> a, T c, T s)
+{
+    return {a.x * c - a.y * s, a.x * s + a.y * c};
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(cplx<T>& a, cplx<T>& b, cplx<T>& c, cplx<T>& d, T sg)
+{
+    const cplx<T> t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = csub(b, d);
+    const cplx<T> it3 = cis(t3, sg);
+    a = cadd(t0, t2);
+    b = cadd(t1, it3);
+    c = csub(t0, t2);
+    d = csub(t1, it3);
+}
+
 template <typename T, int R>
 __device__ __forceinline__ void bfly(cplx<T>* v, T sg)
 {
@@ -190,12 +226,45 @@ __device__ __forceinline__ void bfly(cplx<T>* v, T sg)
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
     } else if constexpr (R == 4) {
-        const cplx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]), t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
-        const cplx<T> it3 = cis(t3, sg);
-        v[0] = cadd(t0, t2);
-        v[1] = cadd(t1, it3);
-        v[2] = csub(t0, t2);
-        v[3] = csub(t1, it3);
+        dft4(v[0], v[1], v[2], v[3], sg);
+    } else if constexpr (R == 8) {
+        // n = 2 n1 + n2, k = k1 + 4 k2: DFT4 over n1, inner twiddle W8^(n2 k1), DFT2 over n2
+        const T h = (T)0.70710678118654752440;
+        cplx<T> a0 = v[0], a1 = v[2], a2 = v[4], a3 = v[6];
+        cplx<T> b0 = v[1], b1 = v[3], b2 = v[5], b3 = v[7];
+        dft4(a0, a1, a2, a3, sg);
+        dft4(b0, b1, b2, b3, sg);
+        b1 = crot(b1, h, sg * h);      // W8^1
+        b2 = cis(b2, sg);              // W8^2 = i sg
+        b3 = crot(b3, -h, sg * h);     // W8^3
+        v[0] = cadd(a0, b0); v[4] = csub(a0, b0);
+        v[1] = cadd(a1, b1); v[5] = csub(a1, b1);
+        v[2] = cadd(a2, b2); v[6] = csub(a2, b2);
+        v[3] = cadd(a3, b3); v[7] = csub(a3, b3);
+    } else if constexpr (R == 16) {
+        // n = 4 n1 + n2, k = k1 + 4 k2: DFT4 over n1, inner twiddle W16^(n2 k1), DFT4 over n2
+        const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
+        cplx<T> a[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            a[n2][0] = v[n2]; a[n2][1] = v[4 + n2]; a[n2][2] = v[8 + n2]; a[n2][3] = v[12 + n2];
+            dft4(a[n2][0], a[n2][1], a[n2][2], a[n2][3], sg);
+        }
+        a[1][1] = crot(a[1][1], c1, sg * s1);     // W16^1
+        a[1][2] = crot(a[1][2], h, sg * h);       // W16^2
+        a[1][3] = crot(a[1][3], s1, sg * c1);     // W16^3
+        a[2][1] = crot(a[2][1], h, sg * h);       // W16^2
+        a[2][2] = cis(a[2][2], sg);               // W16^4
+        a[2][3] = crot(a[2][3], -h, sg * h);      // W16^6
+        a[3][1] = crot(a[3][1], s1, sg * c1);     // W16^3
+        a[3][2] = crot(a[3][2], -h, sg * h);      // W16^6
+        a[3][3] = crot(a[3][3], -c1, -sg * s1);   // W16^9
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            cplx<T> p = a[0][k1], q = a[1][k1], r = a[2][k1], s = a[3][k1];
+            dft4(p, q, r, s, sg);
+            v[k1] = p; v[k1 + 4] = q; v[k1 + 8] = r; v[k1 + 12] = s;
+        }
     } else if constexpr (R == 3) {
         const T s3 = (T)0.86602540378443864676;
         const cplx<T> a = v[0], t1 = cadd(v[1], v[2]), t3 = csub(v[1], v[2]);
@@ -222,127 +291,165 @@ __device__ __forceinline__ void bfly(cplx<T>* v, T sg)
     }
 }
 
-// LDS index with one pad element per 16: the stride-R stores of a Stockham
-// stage then spread over all banks instead of a quarter of them.
+// LDS index with one pad element per 16: the stride-R accesses of a Stockham
+// stage then spread over the banks instead of a few of them.
 __host__ __device__ constexpr int padi(int i) { return i + (i >> 4); }
-// padded LDS line length of the compile-time path, the plain length otherwise
-__host__ __device__ constexpr int lds_line(int N, int n) { return N > 0 ? padi(N - 1) + 1 : n; }
-
-template <typename T, int N, int ST, int NS>
-__device__ __forceinline__ cplx<T>* ct_stages(cplx<T>* in, cplx<T>* out, int nlines, int ldl,
-                                              const cplx<T>* __restrict__ tw, int sign)
+// LDS line pitch (complex elements): long enough for the padded line and
+// = 1 mod 32, so that threads walking across lines hit distinct banks.
+__host__ __device__ constexpr int lds_pitch(int N, int n)
 {
-    if constexpr (ST >= ct_nst(N)) {
-        return in;
-    } else {
+    const int len = N > 0 ? padi(N - 1) + 1 : n;
+    return ((len + 30) / 32) * 32 + 1;
+}
+
+// Runs the compile-time plan of N on nlines lines with one butterfly per
+// thread per stage (nlines * ct_max_nb(N) <= blockDim.x).  Stage 0 reads its
+// inputs through ld(l, i) -- global memory, or the LDS buffer itself when
+// SRC_BUF; the other stages read buf.  Stages write back into buf in place
+// (all reads of a stage complete before its writes), except that the last
+// stage writes through sf(l, i, v) when DST_GLOBAL.  Every thread of the
+// block must call it; the result is in buf (padded index) unless DST_GLOBAL.
+template <typename T, int N, int ST, int NS, bool SRC_BUF, bool DST_GLOBAL, class LD, class SF>
+__device__ __forceinline__ void ct_run(const LD& ld, const SF& sf, cplx<T>* buf, int nlines, int ldl,
+                                       const cplx<T>* __restrict__ tw, int sign)
+{
+    constexpr int NST = ct_nst(N);
+    if constexpr (ST < NST) {
         constexpr int R = ct_radix(N, ST);
         constexpr int NB = N / R;
         constexpr int TSTEP = N / (NS * R);
+        constexpr bool from_buf = (ST > 0) || SRC_BUF;
+        constexpr bool to_global = DST_GLOBAL && (ST == NST - 1);
         const T sg = (T)sign;
-        for (int w = threadIdx.x; w < nlines * NB; w += blockDim.x) {
-            const int l = w / NB, j = w - l * NB, k = j % NS;
-            const cplx<T>* src = in + l * ldl;
-            cplx<T>* dst = out + l * ldl;
-            const int o = (j / NS) * NS * R + k;
-            cplx<T> v[R];
-            v[0] = src[padi(j)];
+        const int w = threadIdx.x;
+        const bool act = w < nlines * NB;
+        const int l = w / NB, j = w - l * NB, k = j % NS;
+        cplx<T> v[R];
+        if (act) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                if constexpr (NS == 1) v[r] = src[padi(j + r * NB)];
-                else v[r] = cmul(src[padi(j + r * NB)], root(tw, r * k * TSTEP, sign));
+            for (int r = 0; r < R; ++r) {
+                cplx<T> x;
+                if constexpr (ST == 0) x = ld(l, j + r * NB);
+                else x = buf[l * ldl + padi(j + r * NB)];
+                if constexpr (NS > 1) {
+                    if (r > 0) x = cmul(x, root(tw, r * k * TSTEP, sign));
+                }
+                v[r] = x;
             }
-            bfly<T, R>(v, sg);
-#pragma unroll
-            for (int r = 0; r < R; ++r) dst[padi(o + r * NS)] = v[r];
         }
-        __syncthreads();
-        return ct_stages<T, N, ST + 1, NS * R>(out, in, nlines, ldl, tw, sign);
+        if constexpr (from_buf && !to_global) __syncthreads();
+        if (act) {
+            bfly<T, R>(v, sg);
+            const int o = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (to_global) sf(l, o + r * NS, v[r]);
+                else buf[l * ldl + padi(o + r * NS)] = v[r];
+            }
+        }
+        if constexpr (!to_global) __syncthreads();
+        ct_run<T, N, ST + 1, NS * R, SRC_BUF, DST_GLOBAL>(ld, sf, buf, nlines, ldl, tw, sign);
     }
 }
 
-// N > 0: compile-time size; N == 0: runtime plan pl.
-template <typename T, int N>
-__device__ __forceinline__ cplx<T>* run_fft(cplx<T>* A, cplx<T>* B, int n, int nlines, int ldl, const FftPlan& pl,
-                                            const cplx<T>* tw, int sign)
-{
-    if constexpr (N > 0) {
-        (void)n;
-        (void)pl;
-        return ct_stages<T, N, 0, 1>(A, B, nlines, ldl, tw, sign);
-    } else {
-        return lds_fft<T>(A, B, n, nlines, ldl, pl, tw, sign);
-    }
-}
+// Spectral layout (both precisions): [z-1][kx][ky], ky fastest, kx in
+// [0, nx/2]: the y transforms run over contiguous lines and the z solve over
+// contiguous modes m = kx*ny + ky.
 
 // (1a) R2C along x of s = -q/eps on interior rows, two real rows per complex
-// line.  Rows are numbered r = (z-1)*ny + y; line l holds rows 2l and 2l+1.
-// LDS: twiddles (nx), then two line buffers of lpb*nx.
+// line.  A block takes 2*lpb consecutive rows y0.. of one plane; lpb divides
+// ny/2.  The spectrum is written transposed, 2*lpb consecutive ky per kx.
+// LDS: twiddles (nx), line buffers (lpb lines; two for the runtime plan).
 template <typename T, int N>
-__global__ void __launch_bounds__(256) fftx_fwd_kernel(const T* __restrict__ q, cplx<T>* __restrict__ spec,
+__global__ void __launch_bounds__(256) fftx_fwd_kernel(const PoissonJob* __restrict__ jobs,
                                                        const cplx<T>* __restrict__ tw, FftPlan pl, int nx_rt,
-                                                       int ny, int npairs, int lpb, double eps)
+                                                       int ny, int lpb)
 {
+    const PoissonJob& J = jobs[blockIdx.y];
     const int nx = N > 0 ? N : nx_rt;
     const int nxh = nx / 2 + 1;
-    const int ldl = lds_line(N, nx);
+    const int ldl = lds_pitch(N, nx);
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* tws = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* A = tws + nx;
-    cplx<T>* B = A + lpb * ldl;
+    const int groups = ny / (2 * lpb);
+    const int zi = blockIdx.x / groups;              // interior plane index z-1
+    if (zi >= J.n) return;                           // a shorter box of the batch
+    const int y0 = (blockIdx.x - zi * groups) * 2 * lpb;
+    const T* qp = (const T*)J.q + (size_t)(zi + 1) * nx * ny + (size_t)y0 * nx;
+    cplx<T>* spec = (cplx<T>*)J.spec;
+    const double neps = -1.0 / J.eps;
     for (int i = threadIdx.x; i < nx; i += blockDim.x) tws[i] = tw[i];
-    const int l0 = blockIdx.x * lpb;
-    const int nl = min(lpb, npairs - l0);
-    const int nxy = nx * ny;
-    const double neps = -1.0 / eps;
-    for (int w = threadIdx.x; w < nl * nx; w += blockDim.x) {
-        const int l = w / nx, i = w - l * nx;
-        const int r0 = 2 * (l0 + l);
-        const int z = r0 / ny + 1, y = r0 - (z - 1) * ny;
-        const size_t o0 = (size_t)z * nxy + (size_t)y * nx + i;
-        A[l * ldl + (N > 0 ? padi(i) : i)] = {(T)((double)q[o0] * neps), (T)((double)q[o0 + nx] * neps)};
-    }
     __syncthreads();
-    cplx<T>* res = run_fft<T, N>(A, B, nx, nl, ldl, pl, tws, -1);
-    for (int w = threadIdx.x; w < nl * nxh; w += blockDim.x) {
-        const int l = w / nxh, k = w - l * nxh;
+    cplx<T>* res;
+    if constexpr (N > 0) {
+        auto ld = [&](int l, int i) -> cplx<T> {
+            return {(T)((double)qp[(size_t)(2 * l) * N + i] * neps),
+                    (T)((double)qp[(size_t)(2 * l + 1) * N + i] * neps)};
+        };
+        auto sf = [](int, int, cplx<T>) {};
+        ct_run<T, N, 0, 1, false, false>(ld, sf, A, lpb, ldl, tws, -1);
+        res = A;
+    } else {
+        cplx<T>* B = A + lpb * ldl;
+        for (int w = threadIdx.x; w < lpb * nx; w += blockDim.x) {
+            const int l = w / nx, i = w - l * nx;
+            A[l * ldl + i] = {(T)((double)qp[(size_t)(2 * l) * nx + i] * neps),
+                              (T)((double)qp[(size_t)(2 * l + 1) * nx + i] * neps)};
+        }
+        __syncthreads();
+        res = lds_fft<T>(A, B, nx, lpb, ldl, pl, tws, -1);
+    }
+    const int rows = 2 * lpb;
+    cplx<T>* out = spec + (size_t)zi * nxh * ny + y0;
+    for (int w = threadIdx.x; w < nxh * rows; w += blockDim.x) {
+        const int k = w / rows, r = w - k * rows;
+        const int l = r >> 1;
         const int kn = (nx - k) % nx;
         const cplx<T> zk = res[l * ldl + (N > 0 ? padi(k) : k)];
         const cplx<T> zn = res[l * ldl + (N > 0 ? padi(kn) : kn)];
-        // A_k = (Z_k + conj Z_{n-k})/2, B_k = (Z_k - conj Z_{n-k})/(2i)
-        const cplx<T> ak = {(T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y)};
-        const cplx<T> bk = {(T)0.5 * (zk.y + zn.y), (T)-0.5 * (zk.x - zn.x)};
-        const int r0 = 2 * (l0 + l);
-        spec[(size_t)r0 * nxh + k] = ak;
-        spec[(size_t)(r0 + 1) * nxh + k] = bk;
+        // A_k = (Z_k + conj Z_{n-k})/2 (row 2l), B_k = (Z_k - conj Z_{n-k})/(2i) (row 2l+1)
+        const cplx<T> v = (r & 1) ? cplx<T>{(T)0.5 * (zk.y + zn.y), (T)-0.5 * (zk.x - zn.x)}
+                                  : cplx<T>{(T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y)};
+        out[(size_t)k * ny + r] = v;
     }
 }
 
-// (1b)/(3a) complex DFT along y, in place, for strips of W columns.
-// LDS: twiddles (ny), then two buffers of W*ny.
-template <typename T, int N, int W>
-__global__ void __launch_bounds__(256) ffty_kernel(cplx<T>* __restrict__ spec, const cplx<T>* __restrict__ tw,
-                                                   FftPlan pl, int ny_rt, int nxh, int nstrips, int sign)
+// (1b)/(3a) complex DFT along y, in place, over contiguous lines of ny.
+// A block takes G consecutive lines.
+template <typename T, int N>
+__global__ void __launch_bounds__(256) ffty_kernel(const PoissonJob* __restrict__ jobs, const cplx<T>* __restrict__ tw,
+                                                   FftPlan pl, int ny_rt, int nxh, int G, int sign)
 {
+    const PoissonJob& J = jobs[blockIdx.y];
     const int ny = N > 0 ? N : ny_rt;
-    const int ldl = lds_line(N, ny);
+    const int ldl = lds_pitch(N, ny);
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* tws = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* A = tws + ny;
-    cplx<T>* B = A + W * ldl;
+    const int nlines = J.n * nxh;
+    const int L0 = blockIdx.x * G;
+    if (L0 >= nlines) return;                        // a shorter box of the batch
+    const int nl = min(G, nlines - L0);
+    cplx<T>* base = (cplx<T>*)J.spec + (size_t)L0 * ny;
     for (int i = threadIdx.x; i < ny; i += blockDim.x) tws[i] = tw[i];
-    const int plane = blockIdx.x / nstrips;
-    const int c0 = (blockIdx.x - plane * nstrips) * W;
-    const int nc = min(W, nxh - c0);
-    cplx<T>* base = spec + (size_t)plane * ny * nxh + c0;
-    for (int w = threadIdx.x; w < W * ny; w += blockDim.x) {
-        const int yy = w / W, c = w - yy * W;
-        if (c < nc) A[c * ldl + (N > 0 ? padi(yy) : yy)] = base[(size_t)yy * nxh + c];
-    }
     __syncthreads();
-    cplx<T>* res = run_fft<T, N>(A, B, ny, nc, ldl, pl, tws, sign);
-    for (int w = threadIdx.x; w < W * ny; w += blockDim.x) {
-        const int yy = w / W, c = w - yy * W;
-        if (c < nc) base[(size_t)yy * nxh + c] = res[c * ldl + (N > 0 ? padi(yy) : yy)];
+    if constexpr (N > 0) {
+        auto ld = [&](int l, int i) -> cplx<T> { return base[(size_t)l * N + i]; };
+        auto sf = [&](int l, int i, cplx<T> v) { base[(size_t)l * N + i] = v; };
+        ct_run<T, N, 0, 1, false, true>(ld, sf, A, nl, ldl, tws, sign);
+    } else {
+        cplx<T>* B = A + G * ldl;
+        for (int w = threadIdx.x; w < nl * ny; w += blockDim.x) {
+            const int l = w / ny, i = w - l * ny;
+            A[l * ldl + i] = base[(size_t)l * ny + i];
+        }
+        __syncthreads();
+        cplx<T>* res = lds_fft<T>(A, B, ny, nl, ldl, pl, tws, sign);
+        for (int w = threadIdx.x; w < nl * ny; w += blockDim.x) {
+            const int l = w / ny, i = w - l * ny;
+            base[(size_t)l * ny + i] = res[l * ldl + i];
+        }
     }
 }
 
@@ -359,12 +466,16 @@ __global__ void __launch_bounds__(256) ffty_kernel(cplx<T>* __restrict__ spec, c
 constexpr int ZS_MAXM = 16;
 
 template <typename T, int P>
-__global__ void __launch_bounds__(256) zsolve_kernel(cplx<T>* __restrict__ spec, int nmodes, int n, int nxh, int ny,
-                                                     double kxs, double kys, double dz2, int split_mean,
-                                                     double* __restrict__ phibar, double inv_nxy, double voltage)
+__global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restrict__ jobs, int nmodes, int ny,
+                                                     int split_mean, double inv_nxy)
 {
     constexpr int MB = 256 / P;
     constexpr int MAXM = ZS_MAXM;
+    const PoissonJob& J = jobs[blockIdx.y];
+    cplx<T>* __restrict__ spec = (cplx<T>*)J.spec;
+    double* __restrict__ phibar = J.phibar;
+    const int n = J.n;
+    const double kxs = J.kxs, kys = J.kys, dz2 = J.dz2, voltage = J.voltage;
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);                 // [n][MB]
     double* xch = reinterpret_cast<double*>(tile + (size_t)n * MB);       // [MB][P][6]
@@ -381,7 +492,7 @@ __global__ void __launch_bounds__(256) zsolve_kernel(cplx<T>* __restrict__ spec,
     const int Peff = min(P, n);
     const bool active = (c < nm) && (p < Peff);
     const int k0 = (p * n) / Peff, len = ((p + 1) * n) / Peff - k0;
-    const int kxi = m % nxh, kyi = m / nxh;
+    const int kxi = m / ny, kyi = m - kxi * ny;
     const double kxv = kxi * kxs, kyv = (kyi <= ny / 2 ? kyi : kyi - ny) * kys;
     const double bd = -(2.0 + (kxv * kxv + kyv * kyv) * dz2);
 
@@ -497,185 +608,256 @@ __global__ void __launch_bounds__(256) zsolve_kernel(cplx<T>* __restrict__ spec,
 }
 
 // (3b) C2R along x, two real rows per complex line, scaled by 1/(nx*ny), plus
-// the wall potential (F64) or nothing (FP32S fluctuation).
+// the wall potential (F64) or nothing (FP32S fluctuation).  A block takes
+// 2*lpb consecutive rows y0.. of one plane; the transposed spectrum is read
+// as (row 2l, row 2l+1) pairs, coalesced along ky, and rebuilt in LDS as
+// Z = A + i B with the Hermitian half.
 template <typename T, int N>
-__global__ void __launch_bounds__(256) fftx_inv_kernel(const cplx<T>* __restrict__ spec, T* __restrict__ phi,
+__global__ void __launch_bounds__(256) fftx_inv_kernel(const PoissonJob* __restrict__ jobs,
                                                        const cplx<T>* __restrict__ tw, FftPlan pl, int nx_rt,
-                                                       int ny, int nz, int npairs, int lpb, double voltage,
-                                                       int add_linear)
+                                                       int ny, int lpb, int add_linear)
 {
+    const PoissonJob& J = jobs[blockIdx.y];
     const int nx = N > 0 ? N : nx_rt;
     const int nxh = nx / 2 + 1;
-    const int ldl = lds_line(N, nx);
+    const int ldl = lds_pitch(N, nx);
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* tws = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* A = tws + nx;
-    cplx<T>* B = A + lpb * ldl;
+    const int groups = ny / (2 * lpb);
+    const int zi = blockIdx.x / groups;
+    if (zi >= J.n) return;                           // a shorter box of the batch
+    const int y0 = (blockIdx.x - zi * groups) * 2 * lpb;
+    const int z = zi + 1;
+    const int nz = J.n + 2;
+    const double voltage = J.voltage;
+    const cplx<T>* spec = (const cplx<T>*)J.spec;
+    T* phi = (T*)J.phi;
     for (int i = threadIdx.x; i < nx; i += blockDim.x) tws[i] = tw[i];
-    const int l0 = blockIdx.x * lpb;
-    const int nl = min(lpb, npairs - l0);
-    for (int w = threadIdx.x; w < nl * nx; w += blockDim.x) {
-        const int l = w / nx, k = w - l * nx;
-        const int r0 = 2 * (l0 + l);
-        cplx<T> a, b;
-        if (k < nxh) {
-            a = spec[(size_t)r0 * nxh + k];
-            b = spec[(size_t)(r0 + 1) * nxh + k];
-            if (k == 0 || 2 * k == nx) { a.y = (T)0; b.y = (T)0; }
-        } else {
-            a = spec[(size_t)r0 * nxh + (nx - k)];
-            b = spec[(size_t)(r0 + 1) * nxh + (nx - k)];
-            a.y = -a.y;
-            b.y = -b.y;
+    const cplx<T>* in = spec + (size_t)zi * nxh * ny + y0;
+    for (int w = threadIdx.x; w < nxh * lpb; w += blockDim.x) {
+        const int k = w / lpb, l = w - k * lpb;
+        cplx<T> a = in[(size_t)k * ny + 2 * l], b = in[(size_t)k * ny + 2 * l + 1];
+        if (k == 0 || 2 * k == nx) {
+            a.y = (T)0;
+            b.y = (T)0;
         }
-        // Z = A + i B
+        // Z_k = A_k + i B_k; Z_{n-k} = conj(A_k) + i conj(B_k)
         A[l * ldl + (N > 0 ? padi(k) : k)] = {a.x - b.y, a.y + b.x};
+        if (k > 0 && 2 * k < nx) {
+            const int kn = nx - k;
+            A[l * ldl + (N > 0 ? padi(kn) : kn)] = {a.x + b.y, b.x - a.y};
+        }
     }
     __syncthreads();
-    cplx<T>* res = run_fft<T, N>(A, B, nx, nl, ldl, pl, tws, +1);
     const double inv = 1.0 / ((double)nx * (double)ny);
-    const int nxy = nx * ny;
-    for (int w = threadIdx.x; w < nl * nx; w += blockDim.x) {
-        const int l = w / nx, i = w - l * nx;
-        const int r0 = 2 * (l0 + l);
-        const int z = r0 / ny + 1, y = r0 - (z - 1) * ny;
-        const cplx<T> v = res[l * ldl + (N > 0 ? padi(i) : i)];
-        double p0 = (double)v.x * inv, p1 = (double)v.y * inv;
-        if (add_linear) {
-            const double lin = voltage * (1.0 - (double)z / (double)(nz - 1));
-            p0 += lin;
-            p1 += lin;
+    const double lin = add_linear ? voltage * (1.0 - (double)z / (double)(nz - 1)) : 0.0;
+    T* row0 = phi + (size_t)z * nx * ny + (size_t)y0 * nx;
+    if constexpr (N > 0) {
+        auto ld = [&](int l, int i) -> cplx<T> { return A[l * ldl + padi(i)]; };
+        auto sf = [&](int l, int i, cplx<T> v) {
+            row0[(size_t)(2 * l) * N + i] = (T)((double)v.x * inv + lin);
+            row0[(size_t)(2 * l + 1) * N + i] = (T)((double)v.y * inv + lin);
+        };
+        ct_run<T, N, 0, 1, true, true>(ld, sf, A, lpb, ldl, tws, +1);
+    } else {
+        cplx<T>* B = A + lpb * ldl;
+        cplx<T>* res = lds_fft<T>(A, B, nx, lpb, ldl, pl, tws, +1);
+        for (int w = threadIdx.x; w < lpb * nx; w += blockDim.x) {
+            const int l = w / nx, i = w - l * nx;
+            const cplx<T> v = res[l * ldl + i];
+            row0[(size_t)(2 * l) * nx + i] = (T)((double)v.x * inv + lin);
+            row0[(size_t)(2 * l + 1) * nx + i] = (T)((double)v.y * inv + lin);
         }
-        const size_t o = (size_t)z * nxy + (size_t)y * nx + i;
-        phi[o] = (T)p0;
-        phi[o + nx] = (T)p1;
     }
 }
 
 // Dirichlet values of the potential on the walls (odd_extract, poisson.cu:158-165).
-__global__ void __launch_bounds__(256) phi_walls_kernel(double* phi, int nxy, int nz, double voltage)
+__global__ void __launch_bounds__(256) phi_walls_kernel(const PoissonJob* __restrict__ jobs, int nxy)
 {
+    const PoissonJob& J = jobs[blockIdx.y];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nxy) return;
-    phi[i] = voltage;
-    phi[(size_t)(nz - 1) * nxy + i] = 0.0;
+    double* phi = (double*)J.phi;
+    phi[i] = J.voltage;
+    phi[(size_t)(J.n + 1) * nxy + i] = 0.0;
 }
 
 constexpr size_t kMaxLds = 65536;
 
+// Largest line count per block for an x pass: divides ny/2, one butterfly per
+// thread per stage (compile-time plan), LDS within kMaxLds.
 template <typename T, int N>
-hipError_t launch_ffty(int W, int grid, size_t shm, hipStream_t st, cplx<T>* spec, const cplx<T>* tw,
-                       const FftPlan& pl, int ny, int nxh, int nstrips, int sign)
+int x_lines_per_block(int nx, int ny)
 {
-    switch (W) {
-    case 16: ffty_kernel<T, N, 16><<<grid, 256, shm, st>>>(spec, tw, pl, ny, nxh, nstrips, sign); break;
-    case 8: ffty_kernel<T, N, 8><<<grid, 256, shm, st>>>(spec, tw, pl, ny, nxh, nstrips, sign); break;
-    case 4: ffty_kernel<T, N, 4><<<grid, 256, shm, st>>>(spec, tw, pl, ny, nxh, nstrips, sign); break;
-    case 2: ffty_kernel<T, N, 2><<<grid, 256, shm, st>>>(spec, tw, pl, ny, nxh, nstrips, sign); break;
-    default: return hipErrorInvalidValue;
+    const size_t cs = sizeof(cplx<T>);
+    const int ldl = lds_pitch(N, nx);
+    const int bufs = N > 0 ? 1 : 2;
+    const int cap = N > 0 ? 256 / ct_max_nb(N) : 16;
+    int best = 0;
+    for (int l = 1; l <= ny / 2 && l <= cap; ++l) {
+        if ((ny / 2) % l) continue;
+        if ((size_t)nx * cs + (size_t)bufs * l * ldl * cs <= kMaxLds) best = l;
     }
-    return hipGetLastError();
+    return best;
 }
 
-template <typename T, int NX, int NY>
-hipError_t poisson_sized(const LbmParams& P, const PoissonBufs& pb, const T* q, T* phi, double voltage, double eps,
-                         bool split_mean, hipStream_t st)
+template <typename T, int N>
+int y_lines_per_block(int ny)
 {
-    const int nx = pb.nx, ny = pb.ny, nz = pb.nz, nxh = pb.nxh;
-    const int n = nz - 2;
-    const int npairs = n * ny / 2;
-    const FftPlan plx = make_fft_plan(nx), ply = make_fft_plan(ny);
     const size_t cs = sizeof(cplx<T>);
-    // x passes: lpb complex lines (row pairs) per block, two buffers, ~32 KB
-    const int ldx = lds_line(NX, nx), ldy = lds_line(NY, ny);
-    int lpb = (int)(32768 / (2 * ldx * cs));
-    if (lpb < 1) lpb = 1;
-    if (lpb > npairs) lpb = npairs;
-    const size_t shx = (size_t)nx * cs + 2 * (size_t)lpb * ldx * cs;
-    const int gx = (npairs + lpb - 1) / lpb;
-    // y passes: strips of W columns, the widest (up to 16) whose buffers fit 64 KB
-    int W = 16;
-    while (W > 2 && (size_t)ny * cs + 2 * (size_t)W * ldy * cs > kMaxLds) W /= 2;
-    const int nstrips = (nxh + W - 1) / W;
-    const size_t shy = (size_t)ny * cs + 2 * (size_t)W * ldy * cs;
-    if (shx > kMaxLds || shy > kMaxLds) return hipErrorInvalidValue;
-    const int gy = nstrips * n;
+    const int ldl = lds_pitch(N, ny);
+    const int bufs = N > 0 ? 1 : 2;
+    int g = N > 0 ? 256 / ct_max_nb(N) : 16;
+    while (g > 1 && (size_t)ny * cs + (size_t)bufs * g * ldl * cs > kMaxLds) --g;
+    return ((size_t)ny * cs + (size_t)bufs * g * ldl * cs <= kMaxLds) ? g : 0;
+}
 
-    cplx<T>* spec = (cplx<T>*)pb.spec;
-    const cplx<T>* twx = (const cplx<T>*)pb.twx;
-    const cplx<T>* twy = (const cplx<T>*)pb.twy;
-
-    fftx_fwd_kernel<T, NX><<<gx, 256, shx, st>>>(q, spec, twx, plx, nx, ny, npairs, lpb, eps);
-    hipError_t e = launch_ffty<T, NY>(W, gy, shy, st, spec, twy, ply, ny, nxh, nstrips, -1);
-    if (e != hipSuccess) return e;
-
-    // z solve: chunks per mode P (8, 16 or 32) with chunk length <= ZS_MAXM and the tile under 48 KB
-    const int nmodes = nxh * ny;
-    int Pz = 0;
+// z-solve partition: P (8, 16 or 32) threads per mode with chunks of at most
+// ZS_MAXM and the LDS tile within kMaxLds; fixed by n and the element size so
+// that a box gives the same numbers alone and in a batch.
+int zsolve_partition(int n, size_t cs)
+{
     for (int cand : {8, 16, 32}) {
         const int mb = 256 / cand;
         const size_t lds = (size_t)n * mb * cs + (size_t)mb * cand * 10 * sizeof(double);
-        if ((n + cand - 1) / cand <= ZS_MAXM && lds <= kMaxLds) {
-            Pz = cand;
-            break;
-        }
+        if ((n + cand - 1) / cand <= ZS_MAXM && lds <= kMaxLds) return cand;
     }
-    if (Pz == 0) return hipErrorInvalidValue;
-    const int mb = 256 / Pz;
-    const size_t shz = (size_t)n * mb * cs + (size_t)mb * Pz * 10 * sizeof(double);
-    const int gz = (nmodes + mb - 1) / mb;
-    const double kxs = 2.0 * 3.14159265358979323846 / (nx * P.dx);
-    const double kys = 2.0 * 3.14159265358979323846 / (ny * P.dy);
-    const double dz2 = P.dz * P.dz, inv_nxy = 1.0 / ((double)nx * (double)ny);
-    const int sm = split_mean ? 1 : 0;
-    if (Pz == 8)
-        zsolve_kernel<T, 8><<<gz, 256, shz, st>>>(spec, nmodes, n, nxh, ny, kxs, kys, dz2, sm, pb.phibar, inv_nxy, voltage);
-    else if (Pz == 16)
-        zsolve_kernel<T, 16><<<gz, 256, shz, st>>>(spec, nmodes, n, nxh, ny, kxs, kys, dz2, sm, pb.phibar, inv_nxy, voltage);
-    else
-        zsolve_kernel<T, 32><<<gz, 256, shz, st>>>(spec, nmodes, n, nxh, ny, kxs, kys, dz2, sm, pb.phibar, inv_nxy, voltage);
+    return 0;
+}
 
-    e = launch_ffty<T, NY>(W, gy, shy, st, spec, twy, ply, ny, nxh, nstrips, +1);
-    if (e != hipSuccess) return e;
-    fftx_inv_kernel<T, NX><<<gx, 256, shx, st>>>(spec, phi, twx, plx, nx, ny, nz, npairs, lpb, voltage, split_mean ? 0 : 1);
-    if (!split_mean) phi_walls_kernel<<<(nx * ny + 255) / 256, 256, 0, st>>>((double*)phi, nx * ny, nz, voltage);
+// One Poisson solve for every job of a group (same nx, ny, precision; the
+// jobs' plane counts may differ): each kernel runs once over the group, the
+// job index in blockIdx.y.
+template <typename T, int NX, int NY>
+hipError_t poisson_group(const PoissonGroup& g, hipStream_t st)
+{
+    const int nx = g.nx, ny = g.ny, nxh = nx / 2 + 1;
+    const int nmax = g.max_n;
+    const FftPlan plx = make_fft_plan(nx), ply = make_fft_plan(ny);
+    const size_t cs = sizeof(cplx<T>);
+    const int lpb = g.lpb > 0 ? g.lpb : x_lines_per_block<T, NX>(nx, ny);
+    const int G = g.ylines > 0 ? g.ylines : y_lines_per_block<T, NY>(ny);
+    if (lpb == 0 || G == 0 || (ny / 2) % lpb != 0 || lpb > x_lines_per_block<T, NX>(nx, ny) ||
+        G > y_lines_per_block<T, NY>(ny))
+        return hipErrorInvalidValue;
+    const size_t shx = (size_t)nx * cs + (size_t)(NX > 0 ? 1 : 2) * lpb * lds_pitch(NX, nx) * cs;
+    const size_t shy = (size_t)ny * cs + (size_t)(NY > 0 ? 1 : 2) * G * lds_pitch(NY, ny) * cs;
+    const dim3 gx(nmax * (ny / (2 * lpb)), g.njobs);
+    const dim3 gy((nmax * nxh + G - 1) / G, g.njobs);
+    const cplx<T>* twx = (const cplx<T>*)g.twx;
+    const cplx<T>* twy = (const cplx<T>*)g.twy;
+
+    fftx_fwd_kernel<T, NX><<<gx, 256, shx, st>>>(g.jobs, twx, plx, nx, ny, lpb);
+    ffty_kernel<T, NY><<<gy, 256, shy, st>>>(g.jobs, twy, ply, ny, nxh, G, -1);
+
+    const int nmodes = nxh * ny;
+    const int Pz = zsolve_partition(nmax, cs);
+    if (Pz == 0 || Pz != g.zs_partition) return hipErrorInvalidValue;
+    const int mb = 256 / Pz;
+    const size_t shz = (size_t)nmax * mb * cs + (size_t)mb * Pz * 10 * sizeof(double);
+    const dim3 gz((nmodes + mb - 1) / mb, g.njobs);
+    const double inv_nxy = 1.0 / ((double)nx * (double)ny);
+    const int sm = g.split_mean;
+    if (Pz == 8) zsolve_kernel<T, 8><<<gz, 256, shz, st>>>(g.jobs, nmodes, ny, sm, inv_nxy);
+    else if (Pz == 16) zsolve_kernel<T, 16><<<gz, 256, shz, st>>>(g.jobs, nmodes, ny, sm, inv_nxy);
+    else zsolve_kernel<T, 32><<<gz, 256, shz, st>>>(g.jobs, nmodes, ny, sm, inv_nxy);
+
+    ffty_kernel<T, NY><<<gy, 256, shy, st>>>(g.jobs, twy, ply, ny, nxh, G, +1);
+    fftx_inv_kernel<T, NX><<<gx, 256, shx, st>>>(g.jobs, twx, plx, nx, ny, lpb, sm ? 0 : 1);
+    if (!sm) phi_walls_kernel<<<dim3((nx * ny + 255) / 256, g.njobs), 256, 0, st>>>(g.jobs, nx * ny);
     return hipGetLastError();
 }
 
 // Compile-time FFT sizes for the grids of the configurations; anything else
 // takes the runtime-planned path.
 template <typename T, int NX>
-hipError_t poisson_ny(const LbmParams& P, const PoissonBufs& pb, const T* q, T* phi, double voltage, double eps,
-                      bool split_mean, hipStream_t st)
+hipError_t poisson_ny(const PoissonGroup& g, hipStream_t st)
 {
-    if (pb.ny == NX) return poisson_sized<T, NX, NX>(P, pb, q, phi, voltage, eps, split_mean, st);
-    return poisson_sized<T, NX, 0>(P, pb, q, phi, voltage, eps, split_mean, st);
+    static_assert(ct_complete(NX), "compile-time sizes must factor into 16, 8, 4, 2, 3, 5");
+    if (g.ny == NX) return poisson_group<T, NX, NX>(g, st);
+    return poisson_group<T, NX, 0>(g, st);
 }
 
 template <typename T>
-hipError_t poisson_t(const LbmParams& P, const PoissonBufs& pb, const T* q, T* phi, double voltage, double eps,
-                     bool split_mean, hipStream_t st)
+hipError_t poisson_t(const PoissonGroup& g, hipStream_t st)
 {
-    switch (pb.nx) {
-    case 48: return poisson_ny<T, 48>(P, pb, q, phi, voltage, eps, split_mean, st);
-    case 64: return poisson_ny<T, 64>(P, pb, q, phi, voltage, eps, split_mean, st);
-    case 80: return poisson_ny<T, 80>(P, pb, q, phi, voltage, eps, split_mean, st);
-    case 96: return poisson_ny<T, 96>(P, pb, q, phi, voltage, eps, split_mean, st);
-    case 128: return poisson_ny<T, 128>(P, pb, q, phi, voltage, eps, split_mean, st);
-    case 256: return poisson_ny<T, 256>(P, pb, q, phi, voltage, eps, split_mean, st);
-    case 512: return poisson_ny<T, 512>(P, pb, q, phi, voltage, eps, split_mean, st);
-    default: return poisson_sized<T, 0, 0>(P, pb, q, phi, voltage, eps, split_mean, st);
+    switch (g.nx) {
+    case 48: return poisson_ny<T, 48>(g, st);
+    case 64: return poisson_ny<T, 64>(g, st);
+    case 80: return poisson_ny<T, 80>(g, st);
+    case 96: return poisson_ny<T, 96>(g, st);
+    case 128: return poisson_ny<T, 128>(g, st);
+    case 256: return poisson_ny<T, 256>(g, st);
+    case 512: return poisson_ny<T, 512>(g, st);
+    default: return poisson_group<T, 0, 0>(g, st);
+    }
+}
+
+template <typename T>
+int max_x_lines(int nx, int ny)
+{
+    switch (nx) {
+    case 48: return x_lines_per_block<T, 48>(nx, ny);
+    case 64: return x_lines_per_block<T, 64>(nx, ny);
+    case 80: return x_lines_per_block<T, 80>(nx, ny);
+    case 96: return x_lines_per_block<T, 96>(nx, ny);
+    case 128: return x_lines_per_block<T, 128>(nx, ny);
+    case 256: return x_lines_per_block<T, 256>(nx, ny);
+    case 512: return x_lines_per_block<T, 512>(nx, ny);
+    default: return x_lines_per_block<T, 0>(nx, ny);
+    }
+}
+
+template <typename T>
+int max_y_lines(int nx, int ny)
+{
+    if (ny != nx) return y_lines_per_block<T, 0>(ny);
+    switch (ny) {
+    case 48: return y_lines_per_block<T, 48>(ny);
+    case 64: return y_lines_per_block<T, 64>(ny);
+    case 80: return y_lines_per_block<T, 80>(ny);
+    case 96: return y_lines_per_block<T, 96>(ny);
+    case 128: return y_lines_per_block<T, 128>(ny);
+    case 256: return y_lines_per_block<T, 256>(ny);
+    case 512: return y_lines_per_block<T, 512>(ny);
+    default: return y_lines_per_block<T, 0>(ny);
     }
 }
 
 }  // namespace
 
-hipError_t launch_poisson(Prec prec, const LbmParams& P, const PoissonBufs& pb, const void* q, void* phi,
-                          double voltage, double eps, hipStream_t st)
+PoissonJob make_poisson_job(const LbmParams& P, const PoissonBufs& pb, const void* q, void* phi, double voltage,
+                            double eps)
 {
-    if (prec == Prec::F64)
-        return poisson_t<double>(P, pb, (const double*)q, (double*)phi, voltage, eps, false, st);
-    return poisson_t<float>(P, pb, (const float*)q, (float*)phi, voltage, eps, true, st);
+    PoissonJob j{};
+    j.q = q;
+    j.spec = pb.spec;
+    j.phi = phi;
+    j.phibar = pb.phibar;
+    j.n = pb.nz - 2;
+    j.kxs = 2.0 * 3.14159265358979323846 / (pb.nx * P.dx);
+    j.kys = 2.0 * 3.14159265358979323846 / (pb.ny * P.dy);
+    j.dz2 = P.dz * P.dz;
+    j.eps = eps;
+    j.voltage = voltage;
+    return j;
+}
+
+PoissonShape poisson_shape(Prec prec, int nx, int ny, int nz)
+{
+    PoissonShape s{};
+    const bool f64 = prec == Prec::F64;
+    s.max_x_lines = f64 ? max_x_lines<double>(nx, ny) : max_x_lines<float>(nx, ny);
+    s.max_y_lines = f64 ? max_y_lines<double>(nx, ny) : max_y_lines<float>(nx, ny);
+    s.zs_partition = zsolve_partition(nz - 2, f64 ? 16 : 8);
+    return s;
+}
+
+hipError_t launch_poisson_group(Prec prec, const PoissonGroup& g, hipStream_t st)
+{
+    if (prec == Prec::F64) return poisson_t<double>(g, st);
+    return poisson_t<float>(g, st);
 }
 
 }  // namespace ehd

@@ -73,7 +73,7 @@ EXPORTS = (
     "ehd_write_charge", "ehd_read_populations", "ehd_write_populations", "ehd_diagnostics",
     "ehd_set_nusselt_reference", "ehd_profile_enable", "ehd_profile_read", "ehd_profile_reset",
     "ehd_sync", "ehd_last_error", "ehd_abi_version", "ehd_save_state", "ehd_load_state",
-    "ehd_hydrostatic_current", "ehd_run_to_steady",
+    "ehd_hydrostatic_current", "ehd_run_to_steady", "ehd_hydrostatic_reference",
     # host-only file layer, include/ehd_io.h
     "ehd_io_tecplot", "ehd_io_data_end", "ehd_io_read_end", "ehd_io_dmd", "ehd_io_umax", "ehd_io_current",
 )
@@ -114,6 +114,7 @@ def lib() -> C.CDLL:
     L.ehd_sync.argtypes = [vp]
     L.ehd_hydrostatic_current.argtypes = [C.POINTER(Params), dp]
     L.ehd_run_to_steady.argtypes = [vp, C.c_int, C.c_double, C.c_longlong, C.POINTER(Steady)]
+    L.ehd_hydrostatic_reference.argtypes = [vp, C.c_int, C.c_double, C.c_longlong, dp]
     L.ehd_save_state.argtypes = [vp, C.c_char_p]
     L.ehd_load_state.argtypes = [vp, C.c_char_p]
     cs, i_ = C.c_char_p, C.c_int
@@ -277,6 +278,11 @@ class Box:
 
     def sync(self):
         _check(lib().ehd_sync(self._h))
+
+    def hydrostatic_reference(self, check_every: int = 1000, tol: float = 1e-9, max_steps: int = 10**6) -> float:
+        v = C.c_double()
+        _check(lib().ehd_hydrostatic_reference(self._h, int(check_every), float(tol), int(max_steps), C.byref(v)))
+        return v.value
 
     def run_to_steady(self, check_every: int = 1000, tol: float = 1e-6, max_steps: int = 10**6) -> Steady:
         s = Steady()

@@ -164,8 +164,18 @@ int ehd_diagnostics(ehd_box* b, ehd_diag* out);
 int ehd_set_nusselt_reference(ehd_box* b, double current_hs);
 
 /* Collector current of the analytic hydrostatic state sampled on the grid,
- * evaluated as current() (LBM.cu:1646-1671) evaluates it.  Host only. */
+ * evaluated as current() (LBM.cu:1646-1671) evaluates it.  Host only.  The
+ * lattice's own steady no-flow state differs from it at first order in dz
+ * (13% at nz=17, 3.6% at nz=65 for C=10); ehd_hydrostatic_reference gives
+ * that discrete value. */
 int ehd_hydrostatic_current(const ehd_params* p, double* current);
+
+/* Discrete hydrostatic current of the box: runs a 2x2 column of the same nz
+ * and parameters (FP64, no perturbation) until the current changes by less
+ * than tol (relative) over check_every steps, scales it to the box's nx*ny
+ * columns, stores it as the box's Nusselt reference and returns it.  A box
+ * below the convection threshold then settles at Nu = 1. */
+int ehd_hydrostatic_reference(ehd_box* b, int check_every, double tol, long long max_steps, double* current);
 
 /* Steady-state detector: steps the box in chunks of check_every steps and
  * stops once |Nu(after chunk) - Nu(before chunk)| < tol (e.g. 1e-6 over 1000
