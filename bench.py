@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle OpenMP threads (0: OMP_NUM_THREADS, else the cores this process may use)")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch events (value only)")
+    ap.add_argument("--workload", choices=("box", "ensemble"), default="box",
+                    help="box: one 256x256x128 box (the headline line); ensemble: the 48-box sweep in one batch")
     return ap.parse_args()
 
 
@@ -80,12 +82,70 @@ def cpu_baseline(args, workload):
             "sample": f"FP64 oracle, {workload}, {n} coupled steps after 1 warm-up step, {dt:.1f} s"}
 
 
+def ensemble_params(E):
+    """48 boxes of a lateral-wavelength / T / C sweep: nx = ny in {48..128},
+    nz in {32, 64}, T log-spaced 150-600, C in {5, 10, 20}, M = 10, FP32S."""
+    out = []
+    for b in range(48):
+        n = (48, 64, 80, 96, 128)[b % 5]
+        nz = (32, 64)[(b // 5) % 2]
+        T = 150.0 * 4.0 ** (b / 47.0)
+        out.append(E.make_params(n, n, nz, T=T, C_=(5.0, 10.0, 20.0)[b % 3], M=10.0, Fe=4000.0, Pr=10.0, lam=0.25,
+                                 precision=E.FP32S, init_mode=E.INIT_HYDROSTATIC, seed=100 + b, amp=1e-3))
+    return out
+
+
+def run_ensemble(args):
+    import ehd_amd as E
+
+    params = ensemble_params(E)
+    nodes = sum(p.nx * p.ny * p.nz for p in params)
+    with E.Batch(params, flags=E.BATCH_GRAPH | E.BATCH_TUNE) as bt:
+        bt.init()
+        bt.step(args.warmup)
+        t0 = time.perf_counter()
+        bt.step(args.steps)
+        elapsed = time.perf_counter() - t0
+        tu = bt.tuning()
+    # the same boxes stepped one after another (one box per call, no graph)
+    nseq = max(1, args.steps // 10)
+    boxes = [E.Box(p) for p in params]
+    for b in boxes:
+        b.init()
+        b.step(2)
+    for b in boxes:
+        b.sync()
+    t0 = time.perf_counter()
+    for b in boxes:
+        b.step(nseq)
+    for b in boxes:
+        b.sync()
+    seq = time.perf_counter() - t0
+    for b in boxes:
+        b.close()
+    line = {
+        "metric": "ensemble coupled-step MLUPS", "value": round(nodes * args.steps / elapsed / 1e6, 2), "unit": "MLUPS",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"48-box ensemble, {nodes} nodes, FP32S, one batch (graph + autotuned shapes)"},
+        "one_box_at_a_time_MLUPS": round(nodes * nseq / seq / 1e6, 2),
+        "tuning": {"lbm_grid": tu.lbm_grid, "lbm_tiles": tu.lbm_tiles,
+                   "groups": [[tu.group[i].nx, tu.group[i].nboxes, tu.group[i].x_lines, tu.group[i].y_lines]
+                              for i in range(tu.ngroups)]},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the coupled step is "
               "single-GPU in this build", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        run_ensemble(args)
+        return
     import ehd_amd as E
 
     prec = E.FP32S if args.precision == "fp32s" else E.FP64

@@ -1,0 +1,449 @@
+// batch.cpp -- many independent boxes stepped together (include/ehd.h,
+// ehd_batch_*): one persistent LBM launch over all FP32S boxes, the Poisson
+// passes batched per (nx, ny) group, the steady step pair captured in a HIP
+// graph, and an autotuner for the launch shapes.  Every box runs exactly the
+// arithmetic of a single-box step (same node function, same FFT line
+// programs, same z-solve partition), so results are bit-identical.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "box.h"
+
+using namespace ehd;
+
+namespace {
+
+constexpr int kMaxGroups = 16;
+
+struct Group {
+    PoissonGroup g{};
+    PoissonJob* djobs = nullptr;   // device table of the group's jobs
+    std::vector<int> members;
+};
+
+PopBufs pop_bufs_at(ehd_box* b, int cur)
+{
+    const int A = cur, B = 1 - cur;
+    return PopBufs{b->f[A], b->f0[A], b->h[A], b->h0[A], b->f[B], b->f0[B], b->h[B], b->h0[B]};
+}
+
+}  // namespace
+
+struct ehd_batch {
+    std::vector<ehd_box*> boxes;
+    Prec prec = Prec::F32S;
+    hipStream_t st = nullptr;
+    int flags = 0;
+    // FP32S: one item table per buffer parity
+    void* ditems[2] = {nullptr, nullptr};
+    unsigned total_tiles = 0;
+    unsigned lbm_grid = 0;
+    double lbm_ms = 0.0;
+    std::vector<Group> groups;
+    std::vector<double> group_ms;
+    // graph of the steady step pair, per starting parity
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+int batch_lbm(ehd_batch* bt, int cur, bool first, bool ebuf, bool macro)
+{
+    if (bt->prec == Prec::F32S) {
+        HIPCHK(launch_lbm_f32s_batch(bt->ditems[cur], (int)bt->boxes.size(), bt->total_tiles, bt->lbm_grid, first,
+                                     ebuf, macro, bt->st));
+    } else {
+        for (ehd_box* b : bt->boxes)
+            HIPCHK(launch_lbm_f64(b->lp, pop_bufs_at(b, cur), macro_bufs(b), first, ebuf, macro, bt->st));
+    }
+    return EHD_OK;
+}
+
+int batch_poisson(ehd_batch* bt)
+{
+    for (const Group& gr : bt->groups) HIPCHK(launch_poisson_group(bt->prec, gr.g, bt->st));
+    return EHD_OK;
+}
+
+void drop_graphs(ehd_batch* bt)
+{
+    for (auto& e : bt->exec)
+        if (e) {
+            (void)hipGraphExecDestroy(e);
+            e = nullptr;
+        }
+}
+
+// Capture the two steady steps (direct = ebuf = false, no macroscopic
+// output) that start from buffer parity cur.
+int capture_pair(ehd_batch* bt, int cur)
+{
+    HIPCHK(hipStreamBeginCapture(bt->st, hipStreamCaptureModeThreadLocal));
+    int rc = EHD_OK;
+    for (int k = 0; k < 2 && rc == EHD_OK; ++k) {
+        rc = batch_lbm(bt, (cur + k) & 1, false, false, false);
+        if (rc == EHD_OK) rc = batch_poisson(bt);
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(bt->st, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return set_error(EHD_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&bt->exec[cur], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return set_error(EHD_EHIP, "graph instantiation failed: %s", hipGetErrorString(e));
+    return EHD_OK;
+}
+
+void free_batch(ehd_batch* bt)
+{
+    if (bt->st) (void)hipStreamSynchronize(bt->st);
+    drop_graphs(bt);
+    for (void* p : bt->ditems)
+        if (p) (void)hipFree(p);
+    for (Group& g : bt->groups)
+        if (g.djobs) (void)hipFree(g.djobs);
+    for (ehd_box* b : bt->boxes) ehd_destroy(b);
+    if (bt->st) (void)hipStreamDestroy(bt->st);
+}
+
+// Divisors of ny/2 up to the largest line count that fits, largest first.
+std::vector<int> x_candidates(int ny, int maxl)
+{
+    std::vector<int> c;
+    for (int l = maxl; l >= 1 && c.size() < 4; --l)
+        if ((ny / 2) % l == 0) c.push_back(l);
+    return c;
+}
+
+template <class F>
+int time_ms(ehd_batch* bt, int reps, F launch, double* ms)
+{
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = launch();   // warm-up
+    if (rc == EHD_OK) {
+        (void)hipEventRecord(e0, bt->st);
+        for (int r = 0; r < reps && rc == EHD_OK; ++r) rc = launch();
+        (void)hipEventRecord(e1, bt->st);
+        (void)hipEventSynchronize(e1);
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        *ms = t / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+// Autotuner: before the boxes are initialised (their buffers hold no state
+// yet), time the steady-state kernels for each candidate launch shape and
+// keep the fastest.  None of the shapes changes any result.
+int tune(ehd_batch* bt)
+{
+    const int reps = 5;
+    if (bt->prec == Prec::F32S) {
+        const unsigned T = bt->total_tiles;
+        std::vector<unsigned> cands = {0u};
+        for (unsigned g : {1024u, 2048u, 4096u, 8192u})
+            if (g < T) cands.push_back(g);
+        double best = 1e30;
+        unsigned bestg = 0;
+        for (unsigned g : cands) {
+            bt->lbm_grid = g;
+            double ms = 0;
+            int rc = time_ms(bt, reps, [&] { return batch_lbm(bt, 0, false, false, false); }, &ms);
+            if (rc) return rc;
+            if (ms < best) {
+                best = ms;
+                bestg = g;
+            }
+        }
+        bt->lbm_grid = bestg;
+        bt->lbm_ms = best;
+    }
+    bt->group_ms.assign(bt->groups.size(), 0.0);
+    for (size_t gi = 0; gi < bt->groups.size(); ++gi) {
+        Group& gr = bt->groups[gi];
+        const PoissonShape ps = poisson_shape(bt->prec, gr.g.nx, gr.g.ny, gr.g.max_n + 2);
+        double best = 1e30;
+        int bl = 0, by = 0;
+        for (int xl : x_candidates(gr.g.ny, ps.max_x_lines))
+            for (int yl : {ps.max_y_lines, ps.max_y_lines / 2, ps.max_y_lines / 4}) {
+                if (yl < 1) continue;
+                gr.g.lpb = xl;
+                gr.g.ylines = yl;
+                double ms = 0;
+                int rc = time_ms(bt, reps, [&]() -> int {
+                    HIPCHK(launch_poisson_group(bt->prec, gr.g, bt->st));
+                    return EHD_OK;
+                }, &ms);
+                if (rc) return rc;
+                if (ms < best) {
+                    best = ms;
+                    bl = xl;
+                    by = yl;
+                }
+            }
+        gr.g.lpb = bl;
+        gr.g.ylines = by;
+        bt->group_ms[gi] = best;
+        for (int m : gr.members) {
+            bt->boxes[m]->tune_lpb = bl;
+            bt->boxes[m]->tune_ylines = by;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(bt->st));
+    return EHD_OK;
+}
+
+int check_uniform(ehd_batch* bt)
+{
+    const ehd_box* b0 = bt->boxes[0];
+    for (const ehd_box* b : bt->boxes) {
+        if (!b->initialised) return set_error(EHD_ESTATE, "batch box not initialised");
+        if (b->cur != b0->cur || b->direct != b0->direct || b->ebuf != b0->ebuf)
+            return set_error(EHD_ESTATE, "batch boxes are out of step (a box was stepped alone)");
+    }
+    return EHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ehd_batch_create(const ehd_params* params, int nboxes, int flags, ehd_batch** out)
+{
+    if (!out) return set_error(EHD_EINVAL, "null output");
+    *out = nullptr;
+    if (!params || nboxes < 1) return set_error(EHD_EINVAL, "need at least one box");
+    for (int i = 1; i < nboxes; ++i)
+        if (params[i].precision != params[0].precision) return set_error(EHD_EINVAL, "a batch has one precision");
+    ehd_batch* bt = new (std::nothrow) ehd_batch();
+    if (!bt) return set_error(EHD_ENOMEM, "host allocation failed");
+    auto bail = [&](int code) {
+        free_batch(bt);
+        delete bt;
+        return code;
+    };
+    bt->flags = flags;
+    bt->prec = params[0].precision == EHD_FP64 ? Prec::F64 : Prec::F32S;
+    if (hipStreamCreateWithFlags(&bt->st, hipStreamNonBlocking) != hipSuccess)
+        return bail(set_error(EHD_EHIP, "hipStreamCreate failed"));
+    for (int i = 0; i < nboxes; ++i) {
+        ehd_box* b = nullptr;
+        int rc = ehd_create(&params[i], &b);
+        if (rc) return bail(rc);
+        bt->boxes.push_back(b);
+    }
+    // Poisson groups by (nx, ny); the z-solve partition follows n, so boxes of
+    // one (nx, ny) whose plane counts need different partitions get separate groups
+    for (int i = 0; i < nboxes; ++i) {
+        ehd_box* b = bt->boxes[i];
+        const int zp = poisson_shape(b->prec, b->nx, b->ny, b->nz).zs_partition;
+        Group* gr = nullptr;
+        for (Group& g : bt->groups)
+            if (g.g.nx == b->nx && g.g.ny == b->ny && g.g.zs_partition == zp) gr = &g;
+        if (!gr) {
+            if ((int)bt->groups.size() == kMaxGroups) return bail(set_error(EHD_EINVAL, "more than 16 Poisson groups"));
+            bt->groups.emplace_back();
+            gr = &bt->groups.back();
+            gr->g.nx = b->nx;
+            gr->g.ny = b->ny;
+            gr->g.split_mean = b->prec == Prec::F32S ? 1 : 0;
+            gr->g.zs_partition = zp;
+            gr->g.twx = b->twx;
+            gr->g.twy = b->twy;
+        }
+        gr->members.push_back(i);
+        gr->g.max_n = std::max(gr->g.max_n, b->nz - 2);
+    }
+    for (Group& gr : bt->groups) {
+        // the largest box of a group fixes the z-solve tile; its partition must serve every member
+        if (poisson_shape(bt->prec, gr.g.nx, gr.g.ny, gr.g.max_n + 2).zs_partition != gr.g.zs_partition)
+            return bail(set_error(EHD_EINVAL, "boxes of one (nx, ny) need different z-solve partitions"));
+        std::vector<PoissonJob> jobs;
+        for (int m : gr.members) jobs.push_back(bt->boxes[m]->pjob_h);
+        gr.g.njobs = (int)jobs.size();
+        if (hipMalloc((void**)&gr.djobs, jobs.size() * sizeof(PoissonJob)) != hipSuccess)
+            return bail(set_error(EHD_ENOMEM, "job table allocation failed"));
+        if (hipMemcpy(gr.djobs, jobs.data(), jobs.size() * sizeof(PoissonJob), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(set_error(EHD_EHIP, "job table upload failed"));
+        gr.g.jobs = gr.djobs;
+    }
+    if (bt->prec == Prec::F32S) {
+        const size_t isz = lbm_f32s_batch_item_size();
+        for (int cur = 0; cur < 2; ++cur) {
+            std::vector<unsigned char> h(isz * nboxes);
+            unsigned tile0 = 0;
+            for (int i = 0; i < nboxes; ++i) {
+                ehd_box* b = bt->boxes[i];
+                lbm_f32s_batch_item(h.data() + i * isz, b->lp, pop_bufs_at(b, cur), macro_bufs(b), tile0);
+                tile0 += (unsigned)((b->S + 255) / 256);
+            }
+            bt->total_tiles = tile0;
+            if (hipMalloc(&bt->ditems[cur], h.size()) != hipSuccess)
+                return bail(set_error(EHD_ENOMEM, "item table allocation failed"));
+            if (hipMemcpy(bt->ditems[cur], h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return bail(set_error(EHD_EHIP, "item table upload failed"));
+        }
+    }
+    if (flags & EHD_BATCH_TUNE) {
+        int rc = tune(bt);
+        if (rc) return bail(rc);
+    }
+    *out = bt;
+    return EHD_OK;
+}
+
+void ehd_batch_destroy(ehd_batch* bt)
+{
+    if (!bt) return;
+    free_batch(bt);
+    delete bt;
+}
+
+int ehd_batch_init(ehd_batch* bt, const double* const* fields9)
+{
+    if (!bt) return set_error(EHD_EINVAL, "null batch");
+    for (size_t i = 0; i < bt->boxes.size(); ++i) {
+        int rc = ehd_init(bt->boxes[i], fields9 ? fields9[i] : nullptr);
+        if (rc) return rc;
+    }
+    return EHD_OK;
+}
+
+int ehd_batch_step(ehd_batch* bt, int nsteps)
+{
+    if (!bt) return set_error(EHD_EINVAL, "null batch");
+    if (nsteps < 0) return set_error(EHD_EINVAL, "nsteps < 0");
+    if (nsteps == 0) return EHD_OK;
+    int rc = check_uniform(bt);
+    if (rc) return rc;
+    for (ehd_box* b : bt->boxes) HIPCHK(hipStreamSynchronize(b->st));
+    int cur = bt->boxes[0]->cur;
+    bool direct = bt->boxes[0]->direct, ebuf = bt->boxes[0]->ebuf;
+    const bool graph = (bt->flags & EHD_BATCH_GRAPH) != 0;
+    int i = 0;
+    auto advance = [&](int n) {
+        for (ehd_box* b : bt->boxes)
+            for (int k = 0; k < n; ++k) {
+                b->step += 1;
+                b->t = b->t + b->der.dt;   // main.cu:208, one addition per step as a single box does
+            }
+    };
+    while (i < nsteps) {
+        const bool last = (i == nsteps - 1);
+        if (graph && !direct && !ebuf && i + 2 <= nsteps - 1) {
+            if (!bt->exec[cur] && (rc = capture_pair(bt, cur))) return rc;
+            HIPCHK(hipGraphLaunch(bt->exec[cur], bt->st));
+            i += 2;
+            advance(2);
+            continue;
+        }
+        if ((rc = batch_lbm(bt, cur, direct, ebuf, last))) return rc;
+        if ((rc = batch_poisson(bt))) return rc;
+        cur = 1 - cur;
+        direct = false;
+        ebuf = false;
+        i += 1;
+        advance(1);
+    }
+    for (ehd_box* b : bt->boxes) {
+        b->cur = cur;
+        b->direct = direct;
+        b->ebuf = ebuf;
+    }
+    HIPCHK(hipStreamSynchronize(bt->st));
+    return EHD_OK;
+}
+
+int ehd_batch_diagnostics(ehd_batch* bt, ehd_diag* out)
+{
+    if (!bt || !out) return set_error(EHD_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(bt->st));
+    for (size_t i = 0; i < bt->boxes.size(); ++i) {
+        int rc = ehd_diagnostics(bt->boxes[i], &out[i]);
+        if (rc) return rc;
+    }
+    return EHD_OK;
+}
+
+int ehd_batch_run_to_steady(ehd_batch* bt, int check_every, double tol, long long max_steps, ehd_steady* out)
+{
+    if (!bt || !out) return set_error(EHD_EINVAL, "null argument");
+    if (check_every < 1 || !(tol >= 0.0) || max_steps < 0) return set_error(EHD_EINVAL, "bad steady-state criteria");
+    const size_t n = bt->boxes.size();
+    std::vector<ehd_diag> d(n);
+    int rc = ehd_batch_diagnostics(bt, d.data());
+    if (rc) return rc;
+    std::vector<double> prev(n);
+    for (size_t i = 0; i < n; ++i) {
+        prev[i] = d[i].nusselt;
+        out[i] = ehd_steady{max_steps, 0, d[i].nusselt, d[i].current, 0.0};
+    }
+    long long taken = 0;
+    size_t done = 0;
+    while (taken < max_steps && done < n) {
+        const int k = (int)std::min<long long>(check_every, max_steps - taken);
+        if ((rc = ehd_batch_step(bt, k))) return rc;
+        taken += k;
+        if ((rc = ehd_batch_diagnostics(bt, d.data()))) return rc;
+        for (size_t i = 0; i < n; ++i) {
+            const double delta = std::fabs(d[i].nusselt - prev[i]);
+            prev[i] = d[i].nusselt;
+            if (out[i].converged) continue;
+            out[i].nusselt = d[i].nusselt;
+            out[i].current = d[i].current;
+            out[i].delta = delta;
+            if (k == check_every && delta < tol) {
+                out[i].converged = 1;
+                out[i].steps = taken;
+                ++done;
+            }
+        }
+    }
+    for (size_t i = 0; i < n; ++i)
+        if (!out[i].converged) out[i].steps = taken;
+    return EHD_OK;
+}
+
+ehd_box* ehd_batch_box(ehd_batch* bt, int i)
+{
+    if (!bt || i < 0 || i >= (int)bt->boxes.size()) {
+        set_error(EHD_EINVAL, "no box %d in the batch", i);
+        return nullptr;
+    }
+    return bt->boxes[i];
+}
+
+int ehd_batch_tuning(ehd_batch* bt, ehd_tuning* out)
+{
+    if (!bt || !out) return set_error(EHD_EINVAL, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    out->lbm_tiles = (int)bt->total_tiles;
+    out->lbm_grid = (int)(bt->lbm_grid ? bt->lbm_grid : bt->total_tiles);
+    out->lbm_ms = bt->lbm_ms;
+    out->ngroups = (int)bt->groups.size();
+    for (size_t i = 0; i < bt->groups.size(); ++i) {
+        const Group& g = bt->groups[i];
+        const PoissonShape ps = poisson_shape(bt->prec, g.g.nx, g.g.ny, g.g.max_n + 2);
+        out->group[i].nx = g.g.nx;
+        out->group[i].ny = g.g.ny;
+        out->group[i].nboxes = (int)g.members.size();
+        out->group[i].x_lines = g.g.lpb ? g.g.lpb : ps.max_x_lines;
+        out->group[i].y_lines = g.g.ylines ? g.g.ylines : ps.max_y_lines;
+        out->group[i].ms = i < bt->group_ms.size() ? bt->group_ms[i] : 0.0;
+    }
+    return EHD_OK;
+}
+
+}  // extern "C"

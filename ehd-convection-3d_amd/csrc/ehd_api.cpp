@@ -12,9 +12,8 @@
 #include <new>
 #include <vector>
 
-#include "ehd_internal.h"
+#include "box.h"
 #include "fft_plan.h"
-#include "ehd_error.h"
 
 using namespace ehd;
 
@@ -34,56 +33,11 @@ int ehd::set_error(int code, const char* fmt, ...)
 namespace {
 
 
-#define HIPCHK(call)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (call);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return set_error(EHD_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
-    } while (0)
 
 constexpr double kPi = 3.14159265358979323846;
 
 }  // namespace
 
-struct ehd_box {
-    ehd_params prm;
-    ehd_derived der;
-    LbmParams lp;
-    Prec prec;
-    int nx, ny, nz, nxh;
-    size_t S, nxy, esz;
-    hipStream_t st = nullptr;
-    void* f[2] = {nullptr, nullptr};
-    void* f0[2] = {nullptr, nullptr};
-    void* h[2] = {nullptr, nullptr};
-    void* h0[2] = {nullptr, nullptr};
-    int cur = 0;
-    bool direct = true;        // state holds pre-collision populations
-    bool ebuf = true;          // next collide takes E from ex/ey/ez
-    bool initialised = false;
-    void *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *q = nullptr, *phi = nullptr;
-    double* phibar = nullptr;
-    double *ex = nullptr, *ey = nullptr, *ez = nullptr;
-    void* spec = nullptr;
-    void* twx = nullptr;
-    void* twy = nullptr;
-    double* diag_scr = nullptr;
-    double* diag_out = nullptr;
-    double* shift_rs = nullptr;   // FP32S storage shifts (device)
-    double* shift_qs = nullptr;
-    float* shift_f = nullptr;
-    PoissonJob* pjob = nullptr;   // the box's one-entry Poisson job table (device)
-    int tune_lpb = 0, tune_ylines = 0;   // Poisson lines per block (0: largest that fits)
-    long long step = 0;
-    double t = 0.0;
-    double current_hs = 0.0;
-    // profiling
-    bool prof = false;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> lbm_pairs, poi_pairs, step_pairs;
-    double lbm_ms = 0.0, poi_ms = 0.0, step_ms = 0.0;
-    long long lbm_n = 0, poi_n = 0, step_n = 0;
-};
 
 namespace {
 
@@ -203,6 +157,10 @@ void free_box(ehd_box* b)
     if (b->st) (void)hipStreamDestroy(b->st);
 }
 
+}  // namespace
+
+namespace ehd {
+
 PopBufs pop_bufs(ehd_box* b)
 {
     const int A = b->cur, B = 1 - b->cur;
@@ -271,6 +229,10 @@ int run_poisson(ehd_box* b)
     b->ebuf = false;
     return EHD_OK;
 }
+
+}  // namespace ehd
+
+namespace {
 
 // Hydrostatic unipolar-injection profile (D = 0, u = 0), nondimensional:
 // E^2 = E0^2 + 2 C E0 zeta, q = C E0/E, phi = 1 - (E^3 - E0^3)/(3 C E0),
@@ -534,6 +496,7 @@ int ehd_create(const ehd_params* p, ehd_box** out)
         PoissonBufs pb{b->nx, b->ny, b->nz, b->nxh, b->spec, b->twx, b->twy, b->phibar};
         const PoissonJob job = make_poisson_job(b->lp, pb, b->q, b->phi, p->voltage, p->eps);
         if ((rc = alloc((void**)&b->pjob, sizeof(PoissonJob)))) return bail(rc);
+        b->pjob_h = job;
         if (hipMemcpy(b->pjob, &job, sizeof(job), hipMemcpyHostToDevice) != hipSuccess)
             return bail(set_error(EHD_EHIP, "Poisson job upload failed"));
     }

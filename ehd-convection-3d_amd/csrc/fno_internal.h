@@ -1,0 +1,41 @@
+// fno_internal.h -- kernels of the Fourier-neural-operator surrogate.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ehd {
+
+// C(i1,i2)[m][n] = alpha * sum_k A(i1,i2)[m][k] * B(i1,i2)[k][n] + beta * C(i1,i2)[m][n]
+// with A[m][k] at A + i1*a1 + i2*a2 + m*am + k*ak (likewise B, C): FP32 in
+// memory, bf16 matrix-core products, FP32 accumulation.
+struct GemmArgs {
+    int M, N, K;
+    const float* A;
+    long long am, ak;
+    const float* B;
+    long long bk, bn;
+    float* C;
+    long long cm, cn;
+    int nb1, nb2;
+    long long a1, a2, b1, b2, c1, c2;
+    float alpha, beta;
+    int ksplit;        // > 0: batch index i2 takes k in [i2*ksplit, (i2+1)*ksplit) (split-K partial sums)
+};
+
+hipError_t gemm_bf16(const GemmArgs& g, hipStream_t st);
+// dst[i] = beta*dst[i] + sum_j src[j*n + i], j < nslices
+hipError_t sum_slices(const float* src, float* dst, long long n, int nslices, float beta, hipStream_t st);
+// y = x + bias[c] over [B][C][S]; with gelu: u keeps the pre-activation, y = gelu(u)
+hipError_t bias_act(float* y, float* u, const float* bias, int B, int C, long long S, bool gelu, hipStream_t st);
+// g *= gelu'(u)
+hipError_t gelu_bwd(float* g, const float* u, long long n, hipStream_t st);
+// db[c] (+)= sum_{b,s} g[b][c][s]
+hipError_t channel_sum(const float* g, float* db, int B, int C, long long S, bool accumulate, hipStream_t st);
+// grad = 2 (out - tgt) / n, part[mse_blocks()] = partial sums of (out - tgt)^2
+int mse_blocks();
+hipError_t mse(const float* out, const float* tgt, float* grad, double* part, long long n, hipStream_t st);
+// Adam (Kingma & Ba 2015), t counts from 1
+hipError_t adam(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2, float eps,
+                int t, hipStream_t st);
+
+}  // namespace ehd

@@ -67,6 +67,18 @@ class Steady(C.Structure):
                 ("current", C.c_double), ("delta", C.c_double)]
 
 
+class _TuneGroup(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nboxes", C.c_int), ("x_lines", C.c_int),
+                ("y_lines", C.c_int), ("ms", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("lbm_grid", C.c_int), ("lbm_tiles", C.c_int), ("lbm_ms", C.c_double), ("ngroups", C.c_int),
+                ("group", _TuneGroup * 16)]
+
+
+BATCH_GRAPH, BATCH_TUNE = 1, 2
+
 EXPORTS = (
     "ehd_params_default", "ehd_derive", "ehd_compute_parameters", "ehd_create", "ehd_destroy",
     "ehd_init", "ehd_step", "ehd_lbm_step", "ehd_poisson", "ehd_read_fields", "ehd_write_phi",
@@ -74,6 +86,8 @@ EXPORTS = (
     "ehd_set_nusselt_reference", "ehd_profile_enable", "ehd_profile_read", "ehd_profile_reset",
     "ehd_sync", "ehd_last_error", "ehd_abi_version", "ehd_save_state", "ehd_load_state",
     "ehd_hydrostatic_current", "ehd_run_to_steady", "ehd_hydrostatic_reference",
+    "ehd_batch_create", "ehd_batch_destroy", "ehd_batch_init", "ehd_batch_step", "ehd_batch_diagnostics",
+    "ehd_batch_run_to_steady", "ehd_batch_box", "ehd_batch_tuning",
     # host-only file layer, include/ehd_io.h
     "ehd_io_tecplot", "ehd_io_data_end", "ehd_io_read_end", "ehd_io_dmd", "ehd_io_umax", "ehd_io_current",
 )
@@ -124,12 +138,22 @@ def lib() -> C.CDLL:
     L.ehd_io_dmd.argtypes = [cs, C.c_double, i_, i_, i_, dp]
     L.ehd_io_umax.argtypes = [cs, i_, C.c_double, i_, i_, i_, dp]
     L.ehd_io_current.argtypes = [i_, i_, i_, C.c_double, C.c_double, dp, dp]
+    L.ehd_batch_create.argtypes = [C.POINTER(Params), C.c_int, C.c_int, C.POINTER(vp)]
+    L.ehd_batch_destroy.argtypes = [vp]
+    L.ehd_batch_destroy.restype = None
+    L.ehd_batch_init.argtypes = [vp, C.POINTER(dp)]
+    L.ehd_batch_step.argtypes = [vp, C.c_int]
+    L.ehd_batch_diagnostics.argtypes = [vp, C.POINTER(Diag)]
+    L.ehd_batch_run_to_steady.argtypes = [vp, C.c_int, C.c_double, C.c_longlong, C.POINTER(Steady)]
+    L.ehd_batch_box.argtypes = [vp, C.c_int]
+    L.ehd_batch_box.restype = vp
+    L.ehd_batch_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.ehd_last_error.argtypes = []
     L.ehd_last_error.restype = C.c_char_p
     L.ehd_abi_version.argtypes = []
     for name in EXPORTS:
         f = getattr(L, name)
-        if name not in ("ehd_destroy", "ehd_last_error", "ehd_io_current"):
+        if name not in ("ehd_destroy", "ehd_last_error", "ehd_io_current", "ehd_batch_destroy", "ehd_batch_box"):
             f.restype = C.c_int
     L.ehd_io_current.restype = C.c_double
     _lib = L
@@ -190,17 +214,21 @@ def compute_parameters(d: Derived, p: Params):
 class Box:
     """One simulation box on the GPU (owns every device buffer)."""
 
-    def __init__(self, params: Params):
+    def __init__(self, params: Params, handle=None):
         self.params = params
         self.shape = (params.nz, params.ny, params.nx)
         self.S = params.nx * params.ny * params.nz
-        h = C.c_void_p()
-        _check(lib().ehd_create(C.byref(params), C.byref(h)))
-        self._h = h
+        self._owned = handle is None
+        if handle is None:
+            h = C.c_void_p()
+            _check(lib().ehd_create(C.byref(params), C.byref(h)))
+            handle = h
+        self._h = handle
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().ehd_destroy(self._h)
+            if self._owned:
+                lib().ehd_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -342,3 +370,64 @@ def host_current(q, ez, K, dy):
     nz, ny, nx = q.shape
     a, b = (np.ascontiguousarray(v, dtype=np.float64) for v in (q, ez))
     return lib().ehd_io_current(nx, ny, nz, float(K), float(dy), _dp(a), _dp(b))
+
+
+class Batch:
+    """Many boxes stepped together (ehd_batch_*): one persistent LBM launch,
+    Poisson passes batched per (nx, ny), HIP-graph replay, autotuned shapes."""
+
+    def __init__(self, params_list, flags=BATCH_GRAPH | BATCH_TUNE):
+        self.params = list(params_list)
+        arr = (Params * len(self.params))(*self.params)
+        h = C.c_void_p()
+        _check(lib().ehd_batch_create(arr, len(self.params), int(flags), C.byref(h)))
+        self._h = h
+        self.boxes = [Box(p, handle=C.c_void_p(lib().ehd_batch_box(h, i))) for i, p in enumerate(self.params)]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for b in self.boxes:
+                b._h = None
+            lib().ehd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __len__(self):
+        return len(self.boxes)
+
+    def init(self, fields=None):
+        if fields is None:
+            _check(lib().ehd_batch_init(self._h, None))
+            return
+        keep = [None if f is None else np.ascontiguousarray(f, dtype=np.float64) for f in fields]
+        ptrs = (C.POINTER(C.c_double) * len(keep))(*[_dp(a) if a is not None else None for a in keep])
+        _check(lib().ehd_batch_init(self._h, ptrs))
+
+    def step(self, n: int = 1):
+        _check(lib().ehd_batch_step(self._h, int(n)))
+
+    def diagnostics(self):
+        out = (Diag * len(self.boxes))()
+        _check(lib().ehd_batch_diagnostics(self._h, out))
+        return list(out)
+
+    def run_to_steady(self, check_every: int = 1000, tol: float = 1e-6, max_steps: int = 10**6):
+        out = (Steady * len(self.boxes))()
+        _check(lib().ehd_batch_run_to_steady(self._h, int(check_every), float(tol), int(max_steps), out))
+        return list(out)
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        _check(lib().ehd_batch_tuning(self._h, C.byref(t)))
+        return t

@@ -1,0 +1,257 @@
+"""Learned warm starts: data generation with the batched solver, FNO training
+on the GPU, and the steps-to-steady evaluation against cold starts.
+
+Protocol (the acceptance rule of the surrogate):
+  * training states: boxes with (T, C) drawn from the training ranges, run
+    from the hydrostatic start plus seeded noise to a steady state
+    (|dNu| < tol over `check_every` steps) in batches;
+  * targets: the steady u, v, w in units of the drift velocity K V / Lz and
+    the deviations of q and phi from the hydrostatic profile in units of the
+    injected charge q0 and the voltage V;
+  * evaluation: for held-out (T, C) draws, a cold box (hydrostatic start +
+    noise) and a warm box (the network's fields + the same noise) run to
+    steady in one batch; a warm start is accepted if its final Nu is within
+    0.5 % of the cold start's, otherwise the cold start is the answer.  The
+    gain is the ratio of the steps to steady.
+
+Usage (one MI355X):  python -m ehd_amd.surrogate --grid 32 32 16 --train 256 --test 32
+prints one JSON summary line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import (FP32S, INIT_FIELDS, INIT_HYDROSTATIC, Batch, Box, derive, make_params)
+from .fno import FNO
+
+D_IN, D_OUT = 6, 5
+
+
+def box_params(n, T, C, M=10.0, seed=1, init=INIT_HYDROSTATIC, amp=1e-3):
+    nx, ny, nz = n
+    return make_params(nx, ny, nz, T=T, C_=C, M=M, Fe=4000.0, Pr=10.0, lam=0.25, precision=FP32S,
+                       init_mode=init, seed=seed, amp=amp)
+
+
+def features(n, T, C, M=10.0):
+    nx, ny, nz = n
+    a = np.empty((D_IN, nz, ny, nx), np.float32)
+    a[0] = math.log(T / 400.0)
+    a[1] = math.log(C / 10.0)
+    a[2] = math.log(M / 10.0)
+    a[3] = (np.arange(nx) / nx)[None, None, :]
+    a[4] = (np.arange(ny) / ny)[None, :, None]
+    a[5] = (np.arange(nz) / (nz - 1))[:, None, None]
+    return a
+
+
+def hydrostatic_profile(p):
+    """q(z), phi(z) of the analytic hydrostatic state of the box (what the
+    hydrostatic start holds), read back from a freshly initialised box."""
+    q = make_params(p.nx, p.ny, p.nz, T=p.T, C_=p.C, M=p.M, Fe=p.Fe, lam=p.lambda_f, precision=p.precision,
+                    init_mode=INIT_HYDROSTATIC, seed=1, amp=0.0)
+    with Box(q) as b:
+        b.init()
+        f = b.read_fields(("q", "phi"))
+    return f["q"][:, 0, 0].copy(), f["phi"][:, 0, 0].copy()
+
+
+def scales(p):
+    d = derive(p)
+    return d.K * p.voltage / p.lz, d.charge0, p.voltage
+
+
+def targets_from_fields(p, f, prof):
+    U, Q, V = scales(p)
+    qz, phz = prof
+    t = np.empty((D_OUT,) + f["ux"].shape, np.float32)
+    t[0], t[1], t[2] = f["ux"] / U, f["uy"] / U, f["uz"] / U
+    t[3] = (f["q"] - qz[:, None, None]) / Q
+    t[4] = (f["phi"] - phz[:, None, None]) / V
+    # wall rows of the macroscopic outputs are raw collide values: use the extrapolation every consumer uses
+    for k in range(4):
+        t[k, 0] = 2 * t[k, 1] - t[k, 2]
+        t[k, -1] = 2 * t[k, -2] - t[k, -3]
+    return t
+
+
+def fields9_from_prediction(p, y, prof, noise_seed=None, amp=0.0):
+    """ehd_init fields (rho, q, phi, u, v, w, Ex, Ey, Ez) from a network output."""
+    U, Q, V = scales(p)
+    d = derive(p)
+    qz, phz = prof
+    nz, ny, nx = p.nz, p.ny, p.nx
+    f = np.zeros((9, nz, ny, nx))
+    f[0] = p.rho0
+    f[1] = qz[:, None, None] + Q * y[3]
+    f[2] = phz[:, None, None] + V * y[4]
+    f[2][0] = p.voltage
+    f[2][-1] = 0.0
+    f[3], f[4], f[5] = U * y[0], U * y[1], U * y[2]
+    if noise_seed is not None and amp > 0:
+        rng = np.random.default_rng(noise_seed)
+        f[3:6] += amp * rng.uniform(-1, 1, (3, nz, ny, nx))
+    f[3:6, 0] = 0.0
+    f[3:6, -1] = 0.0
+    phi = f[2]
+    f[6] = (np.roll(phi, 1, axis=2) - np.roll(phi, -1, axis=2)) / (2 * d.dx)
+    f[7] = (np.roll(phi, 1, axis=1) - np.roll(phi, -1, axis=1)) / (2 * d.dy)
+    ez = np.empty_like(phi)
+    ez[1:-1] = (phi[:-2] - phi[2:]) / (2 * d.dz)
+    ez[0], ez[-1] = ez[1], ez[-2]
+    f[8] = ez
+    return f
+
+
+class Steady:
+    def __init__(self, steps, converged, nusselt):
+        self.steps, self.converged, self.nusselt = steps, converged, nusselt
+
+
+def run_batch_to_steady(params, fields=None, check_every=1000, tol=1e-5, max_steps=200000, tag=""):
+    """Run boxes to steady state in one batch (|dNu| < tol over check_every
+    steps); returns per box (steps, converged, Nu) and the final fields.
+    Progress goes to stderr every half minute."""
+    n = len(params)
+    with Batch(params) as bt:
+        bt.init(fields)
+        prev = [d.nusselt for d in bt.diagnostics()]
+        res = [Steady(max_steps, False, v) for v in prev]
+        taken, t0, tp = 0, time.perf_counter(), time.perf_counter()
+        while taken < max_steps and not all(r.converged for r in res):
+            k = min(check_every, max_steps - taken)
+            bt.step(k)
+            taken += k
+            for i, d in enumerate(bt.diagnostics()):
+                r = res[i]
+                if not r.converged:
+                    r.nusselt = d.nusselt
+                    if k == check_every and abs(d.nusselt - prev[i]) < tol:
+                        r.converged, r.steps = True, taken
+                prev[i] = d.nusselt
+            if time.perf_counter() - tp > 30:
+                tp = time.perf_counter()
+                print(f"{tag} {taken} steps, {sum(r.converged for r in res)}/{n} steady "
+                      f"({tp - t0:.0f} s)", file=sys.stderr, flush=True)
+        out = [b.read_fields(("q", "phi", "ux", "uy", "uz")) for b in bt.boxes]
+    return res, out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--grid", type=int, nargs=3, default=(32, 32, 16))
+    ap.add_argument("--train", type=int, default=256)
+    ap.add_argument("--test", type=int, default=32)
+    ap.add_argument("--batch-boxes", type=int, default=128, help="boxes per solver batch")
+    ap.add_argument("--check-every", type=int, default=1000)
+    ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--max-steps", type=int, default=150000)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--width", type=int, default=32)
+    ap.add_argument("--modes", type=int, default=12)
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args(argv)
+    n = tuple(args.grid)
+    rng = np.random.default_rng(args.seed)
+    t0 = time.perf_counter()
+
+    def draws(k):
+        return [(float(rng.uniform(170.0, 800.0)), float(math.exp(rng.uniform(math.log(5.0), math.log(20.0)))))
+                for _ in range(k)]
+
+    train_draws, test_draws = draws(args.train), draws(args.test)
+    profs = {}
+
+    def prof_of(p):
+        key = (round(p.C, 12),)
+        if key not in profs:
+            profs[key] = hydrostatic_profile(p)
+        return profs[key]
+
+    # 1. training states from the batched solver
+    X, Y = [], []
+    gen_steps = []
+    for s in range(0, len(train_draws), args.batch_boxes):
+        chunk = train_draws[s:s + args.batch_boxes]
+        params = [box_params(n, T, C, seed=1000 + s + i) for i, (T, C) in enumerate(chunk)]
+        res, fields = run_batch_to_steady(params, check_every=args.check_every, tol=args.tol,
+                                          max_steps=args.max_steps, tag=f"train batch {s // args.batch_boxes}:")
+        for p, (T, C), f, r in zip(params, chunk, fields, res):
+            X.append(features(n, T, C))
+            Y.append(targets_from_fields(p, f, prof_of(p)))
+            gen_steps.append(r.steps)
+        print(f"generated {len(X)} states ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+    X = np.stack(X)
+    Y = np.stack(Y)
+    t_gen = time.perf_counter() - t0
+
+    # 2. training on the GPU (the library's forward/backward kernels, Adam)
+    net = FNO(n[0], n[1], n[2], width=args.width, modes=(args.modes,) * 3, max_batch=8)
+    net.set_params(net.init_params(seed=args.seed))
+    ntr = len(X)
+    bs = 8
+    steps_per_epoch = max(1, ntr // bs)
+    total = args.epochs * steps_per_epoch
+    losses = []
+    t1 = time.perf_counter()
+    k = 0
+    for ep in range(args.epochs):
+        order = rng.permutation(ntr)
+        el = []
+        for j in range(steps_per_epoch):
+            idx = order[j * bs:(j + 1) * bs]
+            lr = args.lr * 0.5 * (1 + math.cos(math.pi * k / total))
+            el.append(net.train_step(X[idx], Y[idx], lr))
+            k += 1
+        losses.append(float(np.mean(el)))
+        if ep % 5 == 0 or ep == args.epochs - 1:
+            print(f"epoch {ep}: loss {losses[-1]:.4e} ({time.perf_counter() - t1:.0f} s)", file=sys.stderr, flush=True)
+    t_train = time.perf_counter() - t1
+
+    # 3. steps to steady: cold (hydrostatic + noise) vs warm (network + the same noise), one batch
+    params, fields = [], []
+    for i, (T, C) in enumerate(test_draws):
+        seed = 50000 + i
+        pc = box_params(n, T, C, seed=seed)
+        y = net.forward(features(n, T, C)[None])[0]
+        pw = box_params(n, T, C, seed=seed, init=INIT_FIELDS)
+        params += [pc, pw]
+        fields += [None, fields9_from_prediction(pw, y, prof_of(pw), noise_seed=seed, amp=1e-3)]
+    t2 = time.perf_counter()
+    res, _ = run_batch_to_steady(params, fields, check_every=args.check_every, tol=args.tol,
+                                 max_steps=args.max_steps, tag="evaluation:")
+    t_eval = time.perf_counter() - t2
+    rows = []
+    for i, (T, C) in enumerate(test_draws):
+        c, w = res[2 * i], res[2 * i + 1]
+        ok = bool(c.converged and w.converged and abs(w.nusselt / c.nusselt - 1.0) < 5e-3)
+        rows.append({"T": round(T, 2), "C": round(C, 3), "cold_steps": c.steps, "warm_steps": w.steps,
+                     "cold_Nu": c.nusselt, "warm_Nu": w.nusselt, "accepted": ok})
+    acc = [r for r in rows if r["accepted"]]
+    speed = [r["cold_steps"] / max(r["warm_steps"], 1) for r in acc]
+    summary = {
+        "grid": list(n), "train_states": int(ntr), "test_draws": len(test_draws),
+        "generation_s": round(t_gen, 1), "training_s": round(t_train, 1), "evaluation_s": round(t_eval, 1),
+        "generation_steps_median": int(np.median(gen_steps)),
+        "final_train_loss": losses[-1], "first_epoch_loss": losses[0],
+        "accepted": len(acc), "acceptance_rate": round(len(acc) / len(rows), 3),
+        "steps_ratio_median_accepted": round(float(np.median(speed)), 3) if speed else None,
+        "steps_ratio_mean_accepted": round(float(np.mean(speed)), 3) if speed else None,
+        "total_cold_steps": int(sum(r["cold_steps"] for r in rows)),
+        "total_steps_with_fallback": int(sum(r["warm_steps"] if r["accepted"] else r["warm_steps"] + r["cold_steps"]
+                                             for r in rows)),
+        "draws": rows,
+    }
+    print(json.dumps(summary), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -201,6 +201,50 @@ int ehd_profile_reset(ehd_box* b);
 
 int ehd_sync(ehd_box* b);
 const char* ehd_last_error(void);
+
+/* ---- batches: many independent boxes stepped together ------------------
+ * Stability maps and hysteresis sweeps run many small boxes of uneven size
+ * and parameters (T, C, M, lateral wavelength via nx, ny).  A batch steps
+ * them together: one persistent launch of the fused LBM kernel over every
+ * box (FP32S; FP64 boxes get one launch each), the Poisson solve batched
+ * per (nx, ny) group (one launch per FFT / z-solve pass for all boxes of the
+ * group), and the steady-state part of the step sequence captured once in a
+ * HIP graph and replayed.  Each box gives the same numbers, bit for bit, as
+ * when stepped alone.  No reference counterpart: the reference runs one box
+ * per process (main.cu). */
+typedef struct ehd_batch ehd_batch;
+
+#define EHD_BATCH_GRAPH 1   /* replay the steady steps from a captured HIP graph */
+#define EHD_BATCH_TUNE  2   /* time the launch shapes at creation and keep the fastest */
+
+/* Launch shapes of a batch (the autotuner's choice or the defaults). */
+typedef struct ehd_tuning {
+    int lbm_grid;             /* workgroups of the persistent LBM launch (= tiles when not persistent) */
+    int lbm_tiles;            /* 256-node tiles over all boxes */
+    double lbm_ms;            /* timed LBM launch (tuning run), 0 if not tuned */
+    int ngroups;              /* Poisson groups (distinct nx, ny) */
+    struct {
+        int nx, ny, nboxes, x_lines, y_lines;   /* lines per workgroup of the x / y FFT passes */
+        double ms;                               /* timed Poisson chain of the group, 0 if not tuned */
+    } group[16];
+} ehd_tuning;
+
+/* nboxes parameter sets (all of one precision; at most 16 distinct (nx, ny)). */
+int ehd_batch_create(const ehd_params* params, int nboxes, int flags, ehd_batch** out);
+void ehd_batch_destroy(ehd_batch* bt);
+/* ehd_init of every box; fields9[i] as ehd_init's argument (fields9 itself may be NULL). */
+int ehd_batch_init(ehd_batch* bt, const double* const* fields9);
+int ehd_batch_step(ehd_batch* bt, int nsteps);
+/* ehd_diagnostics of every box (out: nboxes entries) */
+int ehd_batch_diagnostics(ehd_batch* bt, ehd_diag* out);
+/* steps the batch in chunks of check_every until every box has |dNu| < tol
+ * over a chunk (or max_steps); out[i].steps is the step count at which box i
+ * first met the criterion (max_steps if never). */
+int ehd_batch_run_to_steady(ehd_batch* bt, int check_every, double tol, long long max_steps, ehd_steady* out);
+/* member box i (owned by the batch; valid until ehd_batch_destroy) for the
+ * per-box calls: fields, populations, checkpoints, Nusselt reference. */
+ehd_box* ehd_batch_box(ehd_batch* bt, int i);
+int ehd_batch_tuning(ehd_batch* bt, ehd_tuning* out);
 int ehd_abi_version(void);
 
 #ifdef __cplusplus

@@ -1,0 +1,66 @@
+/*
+ * ehd_fno.h -- C-ABI of the Fourier-neural-operator surrogate (warm starts).
+ *
+ * No counterpart in the reference (which has no learned component).  A 3-D
+ * FNO maps the governing groups of a box and its grid coordinates to the
+ * box's steady fields; the lattice Boltzmann run then starts from that
+ * prediction (ehd_init with EHD_INIT_FIELDS) instead of the hydrostatic
+ * state.  Network (activations [batch][channel][z][y][x], FP32 in memory,
+ * every contraction on the bf16 matrix cores with FP32 accumulation):
+ *   lift     v0 = a Pw + Pb                      d_in -> width
+ *   layer l  v' = gelu(K_l v + v W_l + b_l)      (no gelu after the last)
+ *   K_l      truncated real DFT in z (modes 0..mz-1), complex DFTs in y and x
+ *            (modes 0..m-1 and N-m..N-1), per-mode complex channel mixing
+ *            R_l[kz][ky][kx][in][out], the inverse transforms
+ *   project  out = gelu(v Q1w + Q1b) Q2w + Q2b  width -> d_hidden -> d_out
+ * Loss: mean squared error; optimiser: Adam.
+ *
+ * Flat parameter order (float32): Pw[d_in][width], Pb[width], then per layer
+ * W[width][width], b[width], Rr[mz][2my][2mx][width][width], Ri[same], then
+ * Q1w[width][d_hidden], Q1b[d_hidden], Q2w[d_hidden][d_out], Q2b[d_out].
+ * Host arrays: inputs [batch][d_in][Z][Y][X], outputs [batch][d_out][Z][Y][X].
+ * Return codes as in ehd.h; ehd_last_error() holds the message.
+ */
+#ifndef EHD_FNO_H
+#define EHD_FNO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ehd_fno_config {
+    int X, Y, Z;            /* grid; X, Y even, Z >= 2 */
+    int width;              /* channels of the Fourier layers (32) */
+    int mx, my, mz;         /* retained modes per axis (12, capped at X/2, Y/2, Z/2) */
+    int layers;             /* Fourier layers (4) */
+    int d_in, d_hidden, d_out;   /* 6, 128, 5 */
+    int max_batch;          /* largest batch of one call */
+} ehd_fno_config;
+
+typedef struct ehd_fno ehd_fno;
+
+int ehd_fno_create(const ehd_fno_config* cfg, ehd_fno** out);
+void ehd_fno_destroy(ehd_fno* f);
+long long ehd_fno_num_params(const ehd_fno* f);
+int ehd_fno_set_params(ehd_fno* f, const float* params);
+int ehd_fno_get_params(ehd_fno* f, float* params);
+
+/* out = network(in) for batch <= max_batch samples */
+int ehd_fno_forward(ehd_fno* f, int batch, const float* in, float* out);
+
+/* mean-squared-error loss of network(in) against target and its gradient
+ * with respect to every parameter (flat order; grad may be NULL) */
+int ehd_fno_gradients(ehd_fno* f, int batch, const float* in, const float* target, double* loss, float* grad);
+
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the loss of the batch;
+ * loss (may be NULL) receives the loss before the update */
+int ehd_fno_train_step(ehd_fno* f, int batch, const float* in, const float* target, float lr, double* loss);
+
+/* C = A B for row-major host matrices (M x K, K x N) through the bf16
+ * matrix-core GEMM the network uses (checks of its operand layout) */
+int ehd_fno_gemm(int M, int N, int K, const float* A, const float* B, float* C);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

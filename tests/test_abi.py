@@ -9,7 +9,7 @@ import pytest
 import ehd_amd as E
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("ehd.h", "ehd_io.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("ehd.h", "ehd_io.h", "ehd_fno.h")]
 
 
 def declared_functions():
@@ -21,7 +21,8 @@ def declared_functions():
 
 
 def test_header_declares_the_binding_exports():
-    assert declared_functions() == sorted(E.EXPORTS)
+    from ehd_amd import fno
+    assert declared_functions() == sorted(E.EXPORTS + fno.EXPORTS)
 
 
 def test_library_exports_every_declared_symbol():
