@@ -166,15 +166,29 @@ __device__ cplx<T>* lds_fft(cplx<T>* A, cplx<T>* B, int n, int nlines, int ldl, 
 // and 8 butterflies run in registers (as 4x4 and 4x2 with constant inner
 // twiddles), so a 256-point line takes two stages and one LDS exchange.  The
 // radix schedule, the strides and every index division are constants.
+// Radix of stage st: the power-of-two part 2^p is split into ceil(p/4)
+// balanced stages (each 2..16, larger first: 512 = 8*8*8, 256 = 16*16,
+// 128 = 16*8, 64 = 8*8), then the factors 3 and 5.  Balanced stages keep the
+// butterflies per line (N / radix) small in every stage.
 constexpr int ct_radix(int n, int st)
 {
     if (n <= 0) return 0;
-    int r = n, s = 0;
-    const int pref[6] = {16, 8, 4, 2, 3, 5};
-    for (int i = 0; i < 6; ++i)
-        while (r % pref[i] == 0) {
-            if (s == st) return pref[i];
-            r /= pref[i];
+    int p = 0, r = n;
+    while (r % 2 == 0) {
+        r /= 2;
+        ++p;
+    }
+    const int s2 = (p + 3) / 4;
+    if (st < s2) {
+        const int bits = p / s2 + (st < p % s2 ? 1 : 0);
+        return 1 << bits;
+    }
+    int s = s2;
+    const int odd[2] = {3, 5};
+    for (int i = 0; i < 2; ++i)
+        while (r % odd[i] == 0) {
+            if (s == st) return odd[i];
+            r /= odd[i];
             ++s;
         }
     return 0;
@@ -464,6 +478,7 @@ __global__ void __launch_bounds__(256) ffty_kernel(const PoissonJob* __restrict_
 // last x of every chunk) satisfy a banded system that one thread per mode
 // eliminates in order; the chunks then finish independently.
 constexpr int ZS_MAXM = 16;
+constexpr int ZS_LOADS = 8;
 
 template <typename T, int P>
 __global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restrict__ jobs, int nmodes, int ny,
@@ -482,9 +497,22 @@ __global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restric
     double* res = xch + MB * P * 6;                                        // [MB][P][4]
     const int m0 = blockIdx.x * MB;
     const int nm = min(MB, nmodes - m0);
-    for (int w = threadIdx.x; w < n * MB; w += 256) {
-        const int k = w / MB, c = w - k * MB;
-        if (c < nm) tile[w] = spec[(size_t)k * nmodes + m0 + c];
+    // the tile load keeps ZS_LOADS loads in flight per thread (a plain loop
+    // would wait for each load before its LDS store)
+    for (int w0 = threadIdx.x; w0 < n * MB; w0 += 256 * ZS_LOADS) {
+        cplx<T> tmp[ZS_LOADS];
+#pragma unroll
+        for (int u = 0; u < ZS_LOADS; ++u) {
+            const int w = w0 + u * 256;
+            const int k = w / MB, c = w - k * MB;
+            if (w < n * MB && c < nm) tmp[u] = spec[(size_t)k * nmodes + m0 + c];
+        }
+#pragma unroll
+        for (int u = 0; u < ZS_LOADS; ++u) {
+            const int w = w0 + u * 256;
+            const int c = w % MB;
+            if (w < n * MB && c < nm) tile[w] = tmp[u];
+        }
     }
     __syncthreads();
     const int c = threadIdx.x % MB, p = threadIdx.x / MB;
@@ -635,18 +663,36 @@ __global__ void __launch_bounds__(256) fftx_inv_kernel(const PoissonJob* __restr
     T* phi = (T*)J.phi;
     for (int i = threadIdx.x; i < nx; i += blockDim.x) tws[i] = tw[i];
     const cplx<T>* in = spec + (size_t)zi * nxh * ny + y0;
-    for (int w = threadIdx.x; w < nxh * lpb; w += blockDim.x) {
-        const int k = w / lpb, l = w - k * lpb;
-        cplx<T> a = in[(size_t)k * ny + 2 * l], b = in[(size_t)k * ny + 2 * l + 1];
-        if (k == 0 || 2 * k == nx) {
-            a.y = (T)0;
-            b.y = (T)0;
+    // XI_LOADS row pairs in flight per thread before the LDS stores
+    constexpr int XI_LOADS = 4;
+    for (int w0 = threadIdx.x; w0 < nxh * lpb; w0 += blockDim.x * XI_LOADS) {
+        cplx<T> av[XI_LOADS], bv[XI_LOADS];
+#pragma unroll
+        for (int u = 0; u < XI_LOADS; ++u) {
+            const int w = w0 + u * blockDim.x;
+            if (w < nxh * lpb) {
+                const int k = w / lpb, l = w - k * lpb;
+                av[u] = in[(size_t)k * ny + 2 * l];
+                bv[u] = in[(size_t)k * ny + 2 * l + 1];
+            }
         }
-        // Z_k = A_k + i B_k; Z_{n-k} = conj(A_k) + i conj(B_k)
-        A[l * ldl + (N > 0 ? padi(k) : k)] = {a.x - b.y, a.y + b.x};
-        if (k > 0 && 2 * k < nx) {
-            const int kn = nx - k;
-            A[l * ldl + (N > 0 ? padi(kn) : kn)] = {a.x + b.y, b.x - a.y};
+#pragma unroll
+        for (int u = 0; u < XI_LOADS; ++u) {
+            const int w = w0 + u * blockDim.x;
+            if (w < nxh * lpb) {
+                const int k = w / lpb, l = w - k * lpb;
+                cplx<T> a = av[u], b = bv[u];
+                if (k == 0 || 2 * k == nx) {
+                    a.y = (T)0;
+                    b.y = (T)0;
+                }
+                // Z_k = A_k + i B_k; Z_{n-k} = conj(A_k) + i conj(B_k)
+                A[l * ldl + (N > 0 ? padi(k) : k)] = {a.x - b.y, a.y + b.x};
+                if (k > 0 && 2 * k < nx) {
+                    const int kn = nx - k;
+                    A[l * ldl + (N > 0 ? padi(kn) : kn)] = {a.x + b.y, b.x - a.y};
+                }
+            }
         }
     }
     __syncthreads();
@@ -774,7 +820,7 @@ hipError_t poisson_group(const PoissonGroup& g, hipStream_t st)
 template <typename T, int NX>
 hipError_t poisson_ny(const PoissonGroup& g, hipStream_t st)
 {
-    static_assert(ct_complete(NX), "compile-time sizes must factor into 16, 8, 4, 2, 3, 5");
+    static_assert(ct_complete(NX), "compile-time sizes must factor into powers of two, 3 and 5");
     if (g.ny == NX) return poisson_group<T, NX, NX>(g, st);
     return poisson_group<T, NX, 0>(g, st);
 }
