@@ -46,6 +46,7 @@ struct ehd_box {
     float* shift_f = nullptr;
     ehd::PoissonJob* pjob = nullptr;   // the box's one-entry Poisson job table (device)
     ehd::PoissonJob pjob_h{};          // host copy (batches build their tables from it)
+    void* lbm_items[2] = {nullptr, nullptr};   // FP32S: one-item kernel tables per buffer parity
     int tune_lpb = 0, tune_ylines = 0;   // Poisson lines per block (0: largest that fits)
     long long step = 0;
     double t = 0.0;

@@ -147,7 +147,7 @@ void free_box(ehd_box* b)
     void* ptrs[] = {b->f[0], b->f[1], b->f0[0], b->f0[1], b->h[0], b->h[1], b->h0[0], b->h0[1],
                     b->rho, b->ux, b->uy, b->uz, b->q, b->phi, b->phibar, b->ex, b->ey, b->ez,
                     b->spec, b->twx, b->twy, b->diag_scr, b->diag_out,
-                    b->shift_rs, b->shift_qs, b->shift_f, b->pjob};
+                    b->shift_rs, b->shift_qs, b->shift_f, b->pjob, b->lbm_items[0], b->lbm_items[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& pr : b->lbm_pairs) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
@@ -183,7 +183,8 @@ int run_lbm(ehd_box* b, bool macro)
     }
     hipError_t e;
     if (b->prec == Prec::F64) e = launch_lbm_f64(b->lp, pop_bufs(b), macro_bufs(b), b->direct, b->ebuf, macro, b->st);
-    else e = launch_lbm_f32s(b->lp, pop_bufs(b), macro_bufs(b), b->direct, b->ebuf, macro, b->st);
+    else e = launch_lbm_f32s_batch(b->lbm_items[b->cur], 1, (unsigned)((b->S + 255) / 256), 0, b->direct, b->ebuf,
+                                   macro, b->st);
     if (e != hipSuccess) return set_error(EHD_EHIP, "LBM launch failed: %s", hipGetErrorString(e));
     if (b->prof) {
         HIPCHK(hipEventRecord(e1, b->st));
@@ -499,6 +500,19 @@ int ehd_create(const ehd_params* p, ehd_box** out)
         b->pjob_h = job;
         if (hipMemcpy(b->pjob, &job, sizeof(job), hipMemcpyHostToDevice) != hipSuccess)
             return bail(set_error(EHD_EHIP, "Poisson job upload failed"));
+    }
+    if (b->prec == Prec::F32S) {
+        // one-item tables of the FP32S kernel, one per buffer parity
+        const size_t isz = lbm_f32s_batch_item_size();
+        std::vector<unsigned char> item(isz);
+        for (int c = 0; c < 2; ++c) {
+            const int A = c, B = 1 - c;
+            const PopBufs pb{b->f[A], b->f0[A], b->h[A], b->h0[A], b->f[B], b->f0[B], b->h[B], b->h0[B]};
+            lbm_f32s_batch_item(item.data(), b->lp, pb, macro_bufs(b), 0);
+            if ((rc = alloc(&b->lbm_items[c], isz))) return bail(rc);
+            if (hipMemcpy(b->lbm_items[c], item.data(), isz, hipMemcpyHostToDevice) != hipSuccess)
+                return bail(set_error(EHD_EHIP, "LBM item upload failed"));
+        }
     }
     *out = b;
     return EHD_OK;

@@ -54,10 +54,7 @@ struct MacroBufs {
 
 hipError_t launch_lbm_f64(const LbmParams& P, const PopBufs& pb, const MacroBufs& mb,
                           bool first, bool ebuf, bool macro, hipStream_t st);
-hipError_t launch_lbm_f32s(const LbmParams& P, const PopBufs& pb, const MacroBufs& mb,
-                           bool first, bool ebuf, bool macro, hipStream_t st);
-
-// Batched FP32S LBM: an opaque item per box (lbm_f32s_batch_item writes
+// FP32S LBM over one box or a batch of boxes: an opaque item per box (lbm_f32s_batch_item writes
 // lbm_f32s_batch_item_size() bytes), items in a device array sorted by tile0,
 // tile0 = number of 256-node tiles of the boxes before it.  grid = 0 launches
 // one workgroup per tile; a smaller grid makes the workgroups persistent.

@@ -277,20 +277,13 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
     }
 }
 
-// Blocks run in linear order (x fastest, then y, then z): measured faster on
-// MI355X than a remap that gives each XCD a contiguous slab of planes.
-template <bool FIRST, bool EBUF>
-__global__ void __launch_bounds__(256) lbm_f32s_kernel(LbmParams P, F32Consts c, PopBufs pb, MacroBufs mb,
-                                                       int macro)
-{
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= P.S) return;
-    lbm_f32s_node<FIRST, EBUF>(P, c, pb, mb, macro, s);
-}
-
-// Many boxes of a batch in one persistent launch: the boxes' node ranges are
+// One launch over one box or a batch of boxes: the boxes' node ranges are
 // cut into 256-node tiles numbered across the batch (box i owns tiles
-// tile0[i] .. tile0[i+1]-1) and the workgroups stride over all tiles.
+// tile0[i] .. tile0[i+1]-1) and the workgroups stride over all tiles (one
+// tile each when the grid covers them, persistent otherwise).  Tiles run in
+// linear order (x fastest, then y, then z): measured faster on MI355X than a
+// remap that gives each XCD a contiguous slab of planes.  A single box is a
+// batch of one, so a box gives the same numbers alone and in a batch.
 struct BatchItem {
     LbmParams P;
     F32Consts c;
@@ -300,7 +293,7 @@ struct BatchItem {
 };
 
 template <bool FIRST, bool EBUF>
-__global__ void __launch_bounds__(256) lbm_f32s_batch_kernel(const BatchItem* __restrict__ items, int nitems,
+__global__ void __launch_bounds__(256) lbm_f32s_kernel(const BatchItem* __restrict__ items, int nitems,
                                                              unsigned total_tiles, int macro)
 {
     for (unsigned tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
@@ -368,25 +361,12 @@ hipError_t launch_lbm_f32s_batch(const void* items, int nitems, unsigned total_t
     const BatchItem* it = (const BatchItem*)items;
     const unsigned g = (grid == 0 || grid > total_tiles) ? total_tiles : grid;
     const int m = macro ? 1 : 0;
-    if (first && ebuf) lbm_f32s_batch_kernel<true, true><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
-    else if (first) lbm_f32s_batch_kernel<true, false><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
-    else if (ebuf) lbm_f32s_batch_kernel<false, true><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
-    else lbm_f32s_batch_kernel<false, false><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
+    if (first && ebuf) lbm_f32s_kernel<true, true><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
+    else if (first) lbm_f32s_kernel<true, false><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
+    else if (ebuf) lbm_f32s_kernel<false, true><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
+    else lbm_f32s_kernel<false, false><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
     return hipGetLastError();
 }
 
-hipError_t launch_lbm_f32s(const LbmParams& P, const PopBufs& pb, const MacroBufs& mb, bool first, bool ebuf,
-                           bool macro, hipStream_t st)
-{
-    const F32Consts c = make_f32consts(P);
-    const unsigned nblk = (P.S + 255) / 256;
-    dim3 g(nblk), b(256);
-    int m = macro ? 1 : 0;
-    if (first && ebuf) lbm_f32s_kernel<true, true><<<g, b, 0, st>>>(P, c, pb, mb, m);
-    else if (first) lbm_f32s_kernel<true, false><<<g, b, 0, st>>>(P, c, pb, mb, m);
-    else if (ebuf) lbm_f32s_kernel<false, true><<<g, b, 0, st>>>(P, c, pb, mb, m);
-    else lbm_f32s_kernel<false, false><<<g, b, 0, st>>>(P, c, pb, mb, m);
-    return hipGetLastError();
-}
 
 }  // namespace ehd

@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(256) ffty_kernel(const PoissonJob* __restrict_
 // For each (kx,ky) mode the system is x[k-1] + b x[k] + x[k+1] = dz^2 s[k],
 // k = 0..n-1, x[-1] = x[n] = 0, b = -(2 + k^2 dz^2).  A block holds MB modes
 // (a tile [n][MB] staged in LDS for coalesced access) and P = 256/MB threads
-// per mode, each owning a chunk of at most MAXM consecutive k.  Each thread
+// per mode, each owning a chunk of at most MAXM (<= ZS_MAXM) consecutive k.  Each thread
 // solves with the chunk's own matrix T (Thomas, in registers)
 //   T y = r,  T v = e_first,  (T w = e_last, w = reverse of v for Toeplitz T),
 // so that x = y - x_left v - x_right w.  The 2P interface unknowns (first and
@@ -480,12 +480,11 @@ __global__ void __launch_bounds__(256) ffty_kernel(const PoissonJob* __restrict_
 constexpr int ZS_MAXM = 16;
 constexpr int ZS_LOADS = 8;
 
-template <typename T, int P>
+template <typename T, int P, int MAXM>
 __global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restrict__ jobs, int nmodes, int ny,
                                                      int split_mean, double inv_nxy)
 {
     constexpr int MB = 256 / P;
-    constexpr int MAXM = ZS_MAXM;
     const PoissonJob& J = jobs[blockIdx.y];
     cplx<T>* __restrict__ spec = (cplx<T>*)J.spec;
     double* __restrict__ phibar = J.phibar;
@@ -493,25 +492,35 @@ __global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restric
     const double kxs = J.kxs, kys = J.kys, dz2 = J.dz2, voltage = J.voltage;
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);                 // [n][MB]
-    double* xch = reinterpret_cast<double*>(tile + (size_t)n * MB);       // [MB][P][6]
-    double* res = xch + MB * P * 6;                                        // [MB][P][4]
+    // per-chunk records, fields outermost and modes innermost so that the
+    // threads of a wave (consecutive modes) hit consecutive banks
+    double* xch = reinterpret_cast<double*>(tile + (size_t)n * MB);       // [6][P][MB]
+    double* res = xch + MB * P * 6;                                        // [4][P][MB]
+    auto X = [&](int f, int q, int cc) -> double& { return xch[(f * P + q) * MB + cc]; };
+    auto R = [&](int f, int q, int cc) -> double& { return res[(f * P + q) * MB + cc]; };
     const int m0 = blockIdx.x * MB;
     const int nm = min(MB, nmodes - m0);
     // the tile load keeps ZS_LOADS loads in flight per thread (a plain loop
     // would wait for each load before its LDS store)
     for (int w0 = threadIdx.x; w0 < n * MB; w0 += 256 * ZS_LOADS) {
-        cplx<T> tmp[ZS_LOADS];
+        T tx[ZS_LOADS], ty[ZS_LOADS];
 #pragma unroll
         for (int u = 0; u < ZS_LOADS; ++u) {
             const int w = w0 + u * 256;
             const int k = w / MB, c = w - k * MB;
-            if (w < n * MB && c < nm) tmp[u] = spec[(size_t)k * nmodes + m0 + c];
+            tx[u] = (T)0;
+            ty[u] = (T)0;
+            if (w < n * MB && c < nm) {
+                const cplx<T> v = spec[(size_t)k * nmodes + m0 + c];
+                tx[u] = v.x;
+                ty[u] = v.y;
+            }
         }
 #pragma unroll
         for (int u = 0; u < ZS_LOADS; ++u) {
             const int w = w0 + u * 256;
             const int c = w % MB;
-            if (w < n * MB && c < nm) tile[w] = tmp[u];
+            if (w < n * MB && c < nm) tile[w] = cplx<T>{tx[u], ty[u]};
         }
     }
     __syncthreads();
@@ -556,56 +565,49 @@ __global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restric
                 yli = yi[i];
                 vl = v[i];
             }
-        double* e = xch + (c * P + p) * 6;
-        e[0] = yr[0]; e[1] = yi[0]; e[2] = ylr; e[3] = yli; e[4] = v[0]; e[5] = vl;
+        X(0, p, c) = yr[0]; X(1, p, c) = yi[0]; X(2, p, c) = ylr; X(3, p, c) = yli; X(4, p, c) = v[0];
+        X(5, p, c) = vl;
     }
     __syncthreads();
     if (p == 0 && c < nm) {
-        // interface system: x_l(q-1) = al + be * x_f(q) carried along the chunks
-        double gr[P], gi[P], de[P], alr[P], ali[P], bet[P];
+        // interface system: x_l(q-1) = al + be * x_f(q) carried along the
+        // chunks; the per-chunk coefficients replace the chunk's exchange
+        // record in LDS (registers stay independent of P)
         double ar = 0.0, ai = 0.0, be = 0.0;
-#pragma unroll
-        for (int qq = 0; qq < P; ++qq) {
-            if (qq < Peff) {
-                const double* e = xch + (c * P + qq) * 6;
-                const double vf = e[4], vlq = e[5];
-                const double wf = vlq, wl = vf;   // persymmetric chunk matrix
-                const double den = 1.0 / (1.0 + vf * be);
-                gr[qq] = (e[0] - vf * ar) * den;
-                gi[qq] = (e[1] - vf * ai) * den;
-                de[qq] = -wf * den;
-                const double nar = e[2] - vlq * ar - vlq * be * gr[qq];
-                const double nai = e[3] - vlq * ai - vlq * be * gi[qq];
-                const double nbe = -vlq * be * de[qq] - wl;
-                ar = nar; ai = nai; be = nbe;
-                alr[qq] = ar; ali[qq] = ai; bet[qq] = be;
-            }
+        for (int qq = 0; qq < Peff; ++qq) {
+            const double e0 = X(0, qq, c), e1 = X(1, qq, c), e2 = X(2, qq, c), e3 = X(3, qq, c);
+            const double vf = X(4, qq, c), vlq = X(5, qq, c);
+            const double wf = vlq, wl = vf;   // persymmetric chunk matrix
+            const double den = 1.0 / (1.0 + vf * be);
+            const double gr = (e0 - vf * ar) * den;
+            const double gi = (e1 - vf * ai) * den;
+            const double de = -wf * den;
+            const double nar = e2 - vlq * ar - vlq * be * gr;
+            const double nai = e3 - vlq * ai - vlq * be * gi;
+            const double nbe = -vlq * be * de - wl;
+            ar = nar; ai = nai; be = nbe;
+            X(0, qq, c) = gr; X(1, qq, c) = gi; X(2, qq, c) = de; X(3, qq, c) = ar; X(4, qq, c) = ai;
+            X(5, qq, c) = be;
         }
         double xfr = 0.0, xfi = 0.0;   // x_f(q+1)
-#pragma unroll
-        for (int qq = P - 1; qq >= 0; --qq) {
-            if (qq < Peff) {
-                double* r = res + (c * P + qq) * 4;
-                r[2] = xfr;            // right neighbour x_f(q+1)
-                r[3] = xfi;
-                const double xlr = alr[qq] + bet[qq] * xfr, xli = ali[qq] + bet[qq] * xfi;
-                if (qq + 1 < Peff) {
-                    double* rn = res + (c * P + qq + 1) * 4;
-                    rn[0] = xlr;       // left neighbour of chunk q+1
-                    rn[1] = xli;
-                }
-                const double nxr = gr[qq] + de[qq] * xfr, nxi = gi[qq] + de[qq] * xfi;
-                xfr = nxr;
-                xfi = nxi;
+        for (int qq = Peff - 1; qq >= 0; --qq) {
+            R(2, qq, c) = xfr;     // right neighbour x_f(q+1)
+            R(3, qq, c) = xfi;
+            const double xlr = X(3, qq, c) + X(5, qq, c) * xfr, xli = X(4, qq, c) + X(5, qq, c) * xfi;
+            if (qq + 1 < Peff) {
+                R(0, qq + 1, c) = xlr;   // left neighbour of chunk q+1
+                R(1, qq + 1, c) = xli;
             }
+            const double nxr = X(0, qq, c) + X(2, qq, c) * xfr, nxi = X(1, qq, c) + X(2, qq, c) * xfi;
+            xfr = nxr;
+            xfi = nxi;
         }
-        res[(c * P) * 4 + 0] = 0.0;
-        res[(c * P) * 4 + 1] = 0.0;
+        R(0, 0, c) = 0.0;
+        R(1, 0, c) = 0.0;
     }
     __syncthreads();
     if (active) {
-        const double* r = res + (c * P + p) * 4;
-        const double Lr = r[0], Li = r[1], Rr = r[2], Ri = r[3];
+        const double Lr = R(0, p, c), Li = R(1, p, c), Rr = R(2, p, c), Ri = R(3, p, c);
         double wn = 0.0;
 #pragma unroll
         for (int i = MAXM - 1; i >= 0; --i) {
@@ -805,9 +807,23 @@ hipError_t poisson_group(const PoissonGroup& g, hipStream_t st)
     const dim3 gz((nmodes + mb - 1) / mb, g.njobs);
     const double inv_nxy = 1.0 / ((double)nx * (double)ny);
     const int sm = g.split_mean;
-    if (Pz == 8) zsolve_kernel<T, 8><<<gz, 256, shz, st>>>(g.jobs, nmodes, ny, sm, inv_nxy);
-    else if (Pz == 16) zsolve_kernel<T, 16><<<gz, 256, shz, st>>>(g.jobs, nmodes, ny, sm, inv_nxy);
-    else zsolve_kernel<T, 32><<<gz, 256, shz, st>>>(g.jobs, nmodes, ny, sm, inv_nxy);
+    // chunk registers sized to the longest chunk: MAXM = 16, 8 or 4
+    const int chunk = (nmax + Pz - 1) / Pz;
+    hipError_t ez = hipErrorInvalidValue;
+    if (Pz == 8) {
+        zsolve_kernel<T, 8, 16><<<gz, 256, shz, st>>>(g.jobs, nmodes, ny, sm, inv_nxy);
+        ez = hipSuccess;
+    } else if (Pz == 16) {
+        if (chunk <= 8) zsolve_kernel<T, 16, 8><<<gz, 256, shz, st>>>(g.jobs, nmodes, ny, sm, inv_nxy);
+        else zsolve_kernel<T, 16, 16><<<gz, 256, shz, st>>>(g.jobs, nmodes, ny, sm, inv_nxy);
+        ez = hipSuccess;
+    } else if (Pz == 32) {
+        if (chunk <= 4) zsolve_kernel<T, 32, 4><<<gz, 256, shz, st>>>(g.jobs, nmodes, ny, sm, inv_nxy);
+        else if (chunk <= 8) zsolve_kernel<T, 32, 8><<<gz, 256, shz, st>>>(g.jobs, nmodes, ny, sm, inv_nxy);
+        else zsolve_kernel<T, 32, 16><<<gz, 256, shz, st>>>(g.jobs, nmodes, ny, sm, inv_nxy);
+        ez = hipSuccess;
+    }
+    if (ez != hipSuccess) return ez;
 
     ffty_kernel<T, NY><<<gy, 256, shy, st>>>(g.jobs, twy, ply, ny, nxh, G, +1);
     fftx_inv_kernel<T, NX><<<gx, 256, shx, st>>>(g.jobs, twx, plx, nx, ny, lpb, sm ? 0 : 1);

@@ -14,7 +14,7 @@ Protocol (the acceptance rule of the surrogate):
     0.5 % of the cold start's, otherwise the cold start is the answer.  The
     gain is the ratio of the steps to steady.
 
-Usage (one MI355X):  python -m ehd_amd.surrogate --grid 32 32 16 --train 256 --test 32
+Usage (one MI355X):  python -m ehd_amd.surrogate --grid 64 64 32 --train 128 --test 32
 prints one JSON summary line.
 """
 from __future__ import annotations
@@ -27,7 +27,7 @@ import time
 
 import numpy as np
 
-from . import (FP32S, INIT_FIELDS, INIT_HYDROSTATIC, Batch, Box, derive, make_params)
+from . import (FP32S, INIT_FIELDS, INIT_HYDROSTATIC, Batch, Box, EhdError, derive, make_params)
 from .fno import FNO
 
 D_IN, D_OUT = 6, 5
@@ -112,29 +112,57 @@ def fields9_from_prediction(p, y, prof, noise_seed=None, amp=0.0):
 class Steady:
     def __init__(self, steps, converged, nusselt):
         self.steps, self.converged, self.nusselt = steps, converged, nusselt
+        self.failed = False      # the box went non-finite (the lattice was unstable for it)
 
 
-def run_batch_to_steady(params, fields=None, check_every=1000, tol=1e-5, max_steps=200000, tag=""):
-    """Run boxes to steady state in one batch (|dNu| < tol over check_every
-    steps); returns per box (steps, converged, Nu) and the final fields.
-    Progress goes to stderr every half minute."""
+def _diag(box):
+    try:
+        return box.diagnostics()
+    except EhdError as e:
+        if e.code == -5:     # EHD_ENONFINITE
+            return None
+        raise
+
+
+def run_batch_to_steady(params, fields=None, check_every=1000, tol=1e-5, max_steps=200000, min_steps=0,
+                        confirm=2, tag=""):
+    """Run boxes to steady state in one batch; returns per box (steps,
+    converged, Nu) and the final fields.  A box is steady once, over
+    `confirm` consecutive windows of check_every steps, both |dNu| < tol and
+    |d umax| < tol * max(umax, 1e-6 K V/Lz): near the hydrostatic start Nu
+    hardly moves while the instability grows, but umax changes by its growth
+    rate, so the start cannot pass for steady.  min_steps (optional) delays
+    the test.  Progress goes to stderr every half minute."""
     n = len(params)
+    ufloor = [1e-6 * scales(p)[0] for p in params]
     with Batch(params) as bt:
         bt.init(fields)
-        prev = [d.nusselt for d in bt.diagnostics()]
+        d0 = [_diag(b) for b in bt.boxes]
+        prev = [d.nusselt for d in d0]
+        prevu = [d.umax for d in d0]
         res = [Steady(max_steps, False, v) for v in prev]
+        quiet = [0] * n
         taken, t0, tp = 0, time.perf_counter(), time.perf_counter()
-        while taken < max_steps and not all(r.converged for r in res):
+        while taken < max_steps and not all(r.converged or r.failed for r in res):
             k = min(check_every, max_steps - taken)
             bt.step(k)
             taken += k
-            for i, d in enumerate(bt.diagnostics()):
+            for i, b in enumerate(bt.boxes):
                 r = res[i]
-                if not r.converged:
-                    r.nusselt = d.nusselt
-                    if k == check_every and abs(d.nusselt - prev[i]) < tol:
-                        r.converged, r.steps = True, taken
+                if r.converged or r.failed:
+                    continue
+                d = _diag(b)
+                if d is None:
+                    r.failed, r.steps = True, taken
+                    continue
+                r.nusselt = d.nusselt
+                small = (k == check_every and abs(d.nusselt - prev[i]) < tol
+                         and abs(d.umax - prevu[i]) < tol * max(d.umax, ufloor[i]))
+                quiet[i] = quiet[i] + 1 if small else 0
+                if quiet[i] >= confirm and taken >= min_steps:
+                    r.converged, r.steps = True, taken
                 prev[i] = d.nusselt
+                prevu[i] = d.umax
             if time.perf_counter() - tp > 30:
                 tp = time.perf_counter()
                 print(f"{tag} {taken} steps, {sum(r.converged for r in res)}/{n} steady "
@@ -145,13 +173,14 @@ def run_batch_to_steady(params, fields=None, check_every=1000, tol=1e-5, max_ste
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--grid", type=int, nargs=3, default=(32, 32, 16))
+    ap.add_argument("--grid", type=int, nargs=3, default=(64, 64, 32))
     ap.add_argument("--train", type=int, default=256)
     ap.add_argument("--test", type=int, default=32)
     ap.add_argument("--batch-boxes", type=int, default=128, help="boxes per solver batch")
     ap.add_argument("--check-every", type=int, default=1000)
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--max-steps", type=int, default=150000)
+    ap.add_argument("--min-steps", type=int, default=0)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--width", type=int, default=32)
@@ -182,8 +211,11 @@ def main(argv=None):
         chunk = train_draws[s:s + args.batch_boxes]
         params = [box_params(n, T, C, seed=1000 + s + i) for i, (T, C) in enumerate(chunk)]
         res, fields = run_batch_to_steady(params, check_every=args.check_every, tol=args.tol,
-                                          max_steps=args.max_steps, tag=f"train batch {s // args.batch_boxes}:")
+                                          max_steps=args.max_steps, min_steps=args.min_steps,
+                                          tag=f"train batch {s // args.batch_boxes}:")
         for p, (T, C), f, r in zip(params, chunk, fields, res):
+            if r.failed:
+                continue
             X.append(features(n, T, C))
             Y.append(targets_from_fields(p, f, prof_of(p)))
             gen_steps.append(r.steps)
@@ -226,14 +258,15 @@ def main(argv=None):
         fields += [None, fields9_from_prediction(pw, y, prof_of(pw), noise_seed=seed, amp=1e-3)]
     t2 = time.perf_counter()
     res, _ = run_batch_to_steady(params, fields, check_every=args.check_every, tol=args.tol,
-                                 max_steps=args.max_steps, tag="evaluation:")
+                                 max_steps=args.max_steps, min_steps=0, tag="evaluation:")
     t_eval = time.perf_counter() - t2
     rows = []
     for i, (T, C) in enumerate(test_draws):
         c, w = res[2 * i], res[2 * i + 1]
         ok = bool(c.converged and w.converged and abs(w.nusselt / c.nusselt - 1.0) < 5e-3)
         rows.append({"T": round(T, 2), "C": round(C, 3), "cold_steps": c.steps, "warm_steps": w.steps,
-                     "cold_Nu": c.nusselt, "warm_Nu": w.nusselt, "accepted": ok})
+                     "cold_Nu": c.nusselt, "warm_Nu": w.nusselt, "cold_failed": c.failed,
+                     "warm_failed": w.failed, "accepted": ok})
     acc = [r for r in rows if r["accepted"]]
     speed = [r["cold_steps"] / max(r["warm_steps"], 1) for r in acc]
     summary = {
