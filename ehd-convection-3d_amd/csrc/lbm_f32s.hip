@@ -46,6 +46,16 @@ struct F32Consts {
     float hdx, hdy, hdz;      // 0.5/dx ...
 };
 
+// The buffer pointers of a box come from the item table in memory, so the
+// compiler cannot see that they are global: without the address space it
+// emits flat loads and stores.  Every access goes through these types.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) const float cgfloat;
+typedef __attribute__((address_space(1))) const double cgdouble;
+__device__ __forceinline__ cgfloat* gc(const void* p) { return (cgfloat*)p; }
+__device__ __forceinline__ gfloat* gw(void* p) { return (gfloat*)p; }
+__device__ __forceinline__ cgdouble* gd(const double* p) { return (cgdouble*)p; }
+
 // shift correction of a population of direction d pulled from plane z - c_z
 __device__ __forceinline__ float pulled(float v, int d, float w, float dup, float ddn)
 {
@@ -57,8 +67,8 @@ __device__ __forceinline__ float pulled(float v, int d, float w, float dup, floa
 // Every population is read once and written once per step: streaming
 // (non-temporal) loads and stores keep them from displacing the potential
 // and the shift tables in the caches.
-__device__ __forceinline__ float ldnt(const float* p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ void stnt(float v, float* p) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ float ldnt(cgfloat* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stnt(float v, gfloat* p) { __builtin_nontemporal_store(v, p); }
 
 // One node s of one box: pull, collide, boundary and injection.
 template <bool FIRST, bool EBUF>
@@ -74,14 +84,14 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
     const unsigned YS[3] = {ym * nx, y * nx, yp * nx};
     const unsigned ZS[3] = {zm * nxy, z * nxy, zp * nxy};
 
-    const float* __restrict__ sh = P.shf;
+    cgfloat* __restrict__ sh = gc(P.shf);
     const float rs = sh[z], dRu = sh[nz + z], dRd = sh[2 * nz + z];
     const float qs = sh[3 * nz + z], dQu = sh[4 * nz + z], dQd = sh[5 * nz + z];
 
-    const float* __restrict__ fA = (const float*)pb.f;
-    const float* __restrict__ f0A = (const float*)pb.f0;
-    const float* __restrict__ hA = (const float*)pb.h;
-    const float* __restrict__ h0A = (const float*)pb.h0;
+    cgfloat* __restrict__ fA = gc(pb.f);
+    cgfloat* __restrict__ f0A = gc(pb.f0);
+    cgfloat* __restrict__ hA = gc(pb.h);
+    cgfloat* __restrict__ h0A = gc(pb.h0);
 
     float g[27], k[27];
     g[0] = f0A[s];
@@ -110,16 +120,17 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
     }
 
     // E = -grad(phi): float fluctuation plus FP64 plane means
-    const float* __restrict__ phi = (const float*)mb.phi;
+    cgfloat* __restrict__ phi = gc(mb.phi);
+    cgdouble* __restrict__ phibar = gd(mb.phibar);
     float Ex, Ey, Ez;
     const unsigned yc = y * nx, zc = z * nxy;
     if (EBUF) {
-        Ex = (float)mb.ex[s]; Ey = (float)mb.ey[s]; Ez = (float)mb.ez[s];
+        Ex = (float)gd(mb.ex)[s]; Ey = (float)gd(mb.ey)[s]; Ez = (float)gd(mb.ez)[s];
     } else {
         Ex = c.hdx * (phi[xm + yc + zc] - phi[xp + yc + zc]);
         Ey = c.hdy * (phi[x + ym * nx + zc] - phi[x + yp * nx + zc]);
         const unsigned ze = (z == 0) ? 1u : ((z == nz - 1) ? nz - 2 : z);
-        const float dmn = (float)(mb.phibar[ze - 1] - mb.phibar[ze + 1]);
+        const float dmn = (float)(phibar[ze - 1] - phibar[ze + 1]);
         Ez = c.hdz * ((phi[x + yc + (ze - 1) * nxy] - phi[x + yc + (ze + 1) * nxy]) + dmn);
     }
 
@@ -173,11 +184,11 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
         }
         float Exm, Eym, Ezm;
         if (EBUF) {
-            Exm = (float)mb.ex[s1]; Eym = (float)mb.ey[s1]; Ezm = (float)mb.ez[s1];
+            Exm = (float)gd(mb.ex)[s1]; Eym = (float)gd(mb.ey)[s1]; Ezm = (float)gd(mb.ez)[s1];
         } else {
             Exm = c.hdx * (phi[xm + yc + nxy] - phi[xp + yc + nxy]);
             Eym = c.hdy * (phi[x + ym * nx + nxy] - phi[x + yp * nx + nxy]);
-            const float dmn = (float)(mb.phibar[0] - mb.phibar[2]);
+            const float dmn = (float)(phibar[0] - phibar[2]);
             Ezm = c.hdz * ((phi[x + yc] - phi[x + yc + 2 * nxy]) + dmn);
         }
         const float qm = sh[3 * nz + 1] + dqm;
@@ -198,22 +209,22 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
     }
 
     if (macro) {
-        ((float*)mb.rho)[s] = rho;
-        ((float*)mb.ux)[s] = ux;
-        ((float*)mb.uy)[s] = uy;
-        ((float*)mb.uz)[s] = uz;
+        gw(mb.rho)[s] = rho;
+        gw(mb.ux)[s] = ux;
+        gw(mb.uy)[s] = uy;
+        gw(mb.uz)[s] = uz;
     }
-    ((float*)mb.q)[s] = charge;
+    gw(mb.q)[s] = charge;
 
     const float ucx = ux + c.K * Ex, ucy = uy + c.K * Ey, ucz = uz + c.K * Ez;
     const float U2 = (ux * ux + uy * uy + uz * uz) * c.inv2cs2;
     const float Uc2 = (ucx * ucx + ucy * ucy + ucz * ucz) * c.inv2cs2;
     const float uF = (ux * Fx + uy * Fy + uz * Fz) * c.ics2;
 
-    float* __restrict__ fB = (float*)pb.fo;
-    float* __restrict__ f0B = (float*)pb.f0o;
-    float* __restrict__ hB = (float*)pb.ho;
-    float* __restrict__ h0B = (float*)pb.h0o;
+    gfloat* __restrict__ fB = gw(pb.fo);
+    gfloat* __restrict__ f0B = gw(pb.f0o);
+    gfloat* __restrict__ hB = gw(pb.ho);
+    gfloat* __restrict__ h0B = gw(pb.h0o);
     const bool fwall = (z == 0) || (z == nz - 1);
 
     // rest populations

@@ -35,6 +35,27 @@ struct cplx {
     T x, y;
 };
 
+// Buffer pointers of a solve come from its job table in memory, so the
+// compiler cannot see that they are global: without the address space it
+// emits flat loads and stores.  Kernels access them through gptr().
+#define EHD_GLOBAL __attribute__((address_space(1)))
+template <typename X>
+__device__ __forceinline__ EHD_GLOBAL X* gptr(X* p)
+{
+    return (EHD_GLOBAL X*)p;
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> ldc(EHD_GLOBAL const cplx<T>* p)
+{
+    return cplx<T>{p->x, p->y};
+}
+template <typename T>
+__device__ __forceinline__ void stc(EHD_GLOBAL cplx<T>* p, cplx<T> v)
+{
+    p->x = v.x;
+    p->y = v.y;
+}
+
 template <typename T>
 __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b)
 {
@@ -390,8 +411,8 @@ __global__ void __launch_bounds__(256) fftx_fwd_kernel(const PoissonJob* __restr
     const int zi = blockIdx.x / groups;              // interior plane index z-1
     if (zi >= J.n) return;                           // a shorter box of the batch
     const int y0 = (blockIdx.x - zi * groups) * 2 * lpb;
-    const T* qp = (const T*)J.q + (size_t)(zi + 1) * nx * ny + (size_t)y0 * nx;
-    cplx<T>* spec = (cplx<T>*)J.spec;
+    EHD_GLOBAL const T* qp = gptr((const T*)J.q) + (size_t)(zi + 1) * nx * ny + (size_t)y0 * nx;
+    EHD_GLOBAL cplx<T>* spec = gptr((cplx<T>*)J.spec);
     const double neps = -1.0 / J.eps;
     for (int i = threadIdx.x; i < nx; i += blockDim.x) tws[i] = tw[i];
     __syncthreads();
@@ -415,7 +436,7 @@ __global__ void __launch_bounds__(256) fftx_fwd_kernel(const PoissonJob* __restr
         res = lds_fft<T>(A, B, nx, lpb, ldl, pl, tws, -1);
     }
     const int rows = 2 * lpb;
-    cplx<T>* out = spec + (size_t)zi * nxh * ny + y0;
+    EHD_GLOBAL cplx<T>* out = spec + (size_t)zi * nxh * ny + y0;
     for (int w = threadIdx.x; w < nxh * rows; w += blockDim.x) {
         const int k = w / rows, r = w - k * rows;
         const int l = r >> 1;
@@ -425,7 +446,7 @@ __global__ void __launch_bounds__(256) fftx_fwd_kernel(const PoissonJob* __restr
         // A_k = (Z_k + conj Z_{n-k})/2 (row 2l), B_k = (Z_k - conj Z_{n-k})/(2i) (row 2l+1)
         const cplx<T> v = (r & 1) ? cplx<T>{(T)0.5 * (zk.y + zn.y), (T)-0.5 * (zk.x - zn.x)}
                                   : cplx<T>{(T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y)};
-        out[(size_t)k * ny + r] = v;
+        stc(out + (size_t)k * ny + r, v);
     }
 }
 
@@ -445,24 +466,24 @@ __global__ void __launch_bounds__(256) ffty_kernel(const PoissonJob* __restrict_
     const int L0 = blockIdx.x * G;
     if (L0 >= nlines) return;                        // a shorter box of the batch
     const int nl = min(G, nlines - L0);
-    cplx<T>* base = (cplx<T>*)J.spec + (size_t)L0 * ny;
+    EHD_GLOBAL cplx<T>* base = gptr((cplx<T>*)J.spec) + (size_t)L0 * ny;
     for (int i = threadIdx.x; i < ny; i += blockDim.x) tws[i] = tw[i];
     __syncthreads();
     if constexpr (N > 0) {
-        auto ld = [&](int l, int i) -> cplx<T> { return base[(size_t)l * N + i]; };
-        auto sf = [&](int l, int i, cplx<T> v) { base[(size_t)l * N + i] = v; };
+        auto ld = [&](int l, int i) -> cplx<T> { return ldc(base + (size_t)l * N + i); };
+        auto sf = [&](int l, int i, cplx<T> v) { stc(base + (size_t)l * N + i, v); };
         ct_run<T, N, 0, 1, false, true>(ld, sf, A, nl, ldl, tws, sign);
     } else {
         cplx<T>* B = A + G * ldl;
         for (int w = threadIdx.x; w < nl * ny; w += blockDim.x) {
             const int l = w / ny, i = w - l * ny;
-            A[l * ldl + i] = base[(size_t)l * ny + i];
+            A[l * ldl + i] = ldc(base + (size_t)l * ny + i);
         }
         __syncthreads();
         cplx<T>* res = lds_fft<T>(A, B, ny, nl, ldl, pl, tws, sign);
         for (int w = threadIdx.x; w < nl * ny; w += blockDim.x) {
             const int l = w / ny, i = w - l * ny;
-            base[(size_t)l * ny + i] = res[l * ldl + i];
+            stc(base + (size_t)l * ny + i, res[l * ldl + i]);
         }
     }
 }
@@ -486,8 +507,8 @@ __global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restric
 {
     constexpr int MB = 256 / P;
     const PoissonJob& J = jobs[blockIdx.y];
-    cplx<T>* __restrict__ spec = (cplx<T>*)J.spec;
-    double* __restrict__ phibar = J.phibar;
+    EHD_GLOBAL cplx<T>* __restrict__ spec = gptr((cplx<T>*)J.spec);
+    EHD_GLOBAL double* __restrict__ phibar = gptr(J.phibar);
     const int n = J.n;
     const double kxs = J.kxs, kys = J.kys, dz2 = J.dz2, voltage = J.voltage;
     extern __shared__ unsigned char smem_raw[];
@@ -511,7 +532,7 @@ __global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restric
             tx[u] = (T)0;
             ty[u] = (T)0;
             if (w < n * MB && c < nm) {
-                const cplx<T> v = spec[(size_t)k * nmodes + m0 + c];
+                const cplx<T> v = ldc(spec + (size_t)k * nmodes + m0 + c);
                 tx[u] = v.x;
                 ty[u] = v.y;
             }
@@ -633,7 +654,7 @@ __global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restric
     __syncthreads();
     for (int w = threadIdx.x; w < n * MB; w += 256) {
         const int k = w / MB, cc = w - k * MB;
-        if (cc < nm) spec[(size_t)k * nmodes + m0 + cc] = tile[w];
+        if (cc < nm) stc(spec + (size_t)k * nmodes + m0 + cc, tile[w]);
     }
 }
 
@@ -661,10 +682,10 @@ __global__ void __launch_bounds__(256) fftx_inv_kernel(const PoissonJob* __restr
     const int z = zi + 1;
     const int nz = J.n + 2;
     const double voltage = J.voltage;
-    const cplx<T>* spec = (const cplx<T>*)J.spec;
-    T* phi = (T*)J.phi;
+    EHD_GLOBAL const cplx<T>* spec = gptr((const cplx<T>*)J.spec);
+    EHD_GLOBAL T* phi = gptr((T*)J.phi);
     for (int i = threadIdx.x; i < nx; i += blockDim.x) tws[i] = tw[i];
-    const cplx<T>* in = spec + (size_t)zi * nxh * ny + y0;
+    EHD_GLOBAL const cplx<T>* in = spec + (size_t)zi * nxh * ny + y0;
     // XI_LOADS row pairs in flight per thread before the LDS stores
     constexpr int XI_LOADS = 4;
     for (int w0 = threadIdx.x; w0 < nxh * lpb; w0 += blockDim.x * XI_LOADS) {
@@ -674,8 +695,8 @@ __global__ void __launch_bounds__(256) fftx_inv_kernel(const PoissonJob* __restr
             const int w = w0 + u * blockDim.x;
             if (w < nxh * lpb) {
                 const int k = w / lpb, l = w - k * lpb;
-                av[u] = in[(size_t)k * ny + 2 * l];
-                bv[u] = in[(size_t)k * ny + 2 * l + 1];
+                av[u] = ldc(in + (size_t)k * ny + 2 * l);
+                bv[u] = ldc(in + (size_t)k * ny + 2 * l + 1);
             }
         }
 #pragma unroll
@@ -700,7 +721,7 @@ __global__ void __launch_bounds__(256) fftx_inv_kernel(const PoissonJob* __restr
     __syncthreads();
     const double inv = 1.0 / ((double)nx * (double)ny);
     const double lin = add_linear ? voltage * (1.0 - (double)z / (double)(nz - 1)) : 0.0;
-    T* row0 = phi + (size_t)z * nx * ny + (size_t)y0 * nx;
+    EHD_GLOBAL T* row0 = phi + (size_t)z * nx * ny + (size_t)y0 * nx;
     if constexpr (N > 0) {
         auto ld = [&](int l, int i) -> cplx<T> { return A[l * ldl + padi(i)]; };
         auto sf = [&](int l, int i, cplx<T> v) {
@@ -726,7 +747,7 @@ __global__ void __launch_bounds__(256) phi_walls_kernel(const PoissonJob* __rest
     const PoissonJob& J = jobs[blockIdx.y];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nxy) return;
-    double* phi = (double*)J.phi;
+    EHD_GLOBAL double* phi = gptr((double*)J.phi);
     phi[i] = J.voltage;
     phi[(size_t)(J.n + 1) * nxy + i] = 0.0;
 }
@@ -766,7 +787,9 @@ int y_lines_per_block(int ny)
 // that a box gives the same numbers alone and in a batch.
 int zsolve_partition(int n, size_t cs)
 {
-    for (int cand : {8, 16, 32}) {
+    // 16 threads per mode first (measured fastest at n = 126: 32 us against
+    // 35.6 us for 8 and 62 us for 32 on the 256x256 FP32 spectrum)
+    for (int cand : {16, 8, 32}) {
         const int mb = 256 / cand;
         const size_t lds = (size_t)n * mb * cs + (size_t)mb * cand * 10 * sizeof(double);
         if ((n + cand - 1) / cand <= ZS_MAXM && lds <= kMaxLds) return cand;

@@ -81,6 +81,41 @@ def targets_from_fields(p, f, prof):
     return t
 
 
+def _fold(k, n):
+    return min(k, n - k)
+
+
+def canonicalize(t):
+    """Remove the translation (and, on square boxes, the x/y exchange)
+    symmetry of a steady convection pattern: every periodic shift of a steady
+    state is a steady state, and the network cannot know which phase a run
+    picks.  The dominant Fourier mode of w at mid-height is turned to point
+    along x (swapping x and y, and u and v, if it points along y) and shifted
+    to phase zero by a whole-node circular shift.  t: [5, Z, Y, X] (u, v, w,
+    q, phi) -> the canonical copy."""
+    t = np.array(t, copy=True)
+    Z, Y, X = t.shape[1:]
+    F = np.fft.fft2(t[2, Z // 2])
+    F[0, 0] = 0.0
+    mag = np.abs(F)
+    ky, kx = np.unravel_index(int(np.argmax(mag)), mag.shape)
+    if X == Y and _fold(ky, Y) > _fold(kx, X):
+        t = t.swapaxes(2, 3).copy()
+        t[[0, 1]] = t[[1, 0]]
+        F = np.fft.fft2(t[2, Z // 2])
+        F[0, 0] = 0.0
+        mag = np.abs(F)
+        ky, kx = np.unravel_index(int(np.argmax(mag)), mag.shape)
+    phase = float(np.angle(F[ky, kx]))
+    if kx != 0:
+        shift = int(round(phase * X / (2 * np.pi * kx))) % X
+        t = np.roll(t, shift, axis=3)
+    elif ky != 0:
+        shift = int(round(phase * Y / (2 * np.pi * ky))) % Y
+        t = np.roll(t, shift, axis=2)
+    return t
+
+
 def fields9_from_prediction(p, y, prof, noise_seed=None, amp=0.0):
     """ehd_init fields (rho, q, phi, u, v, w, Ex, Ey, Ez) from a network output."""
     U, Q, V = scales(p)
@@ -180,9 +215,12 @@ def main(argv=None):
     ap.add_argument("--check-every", type=int, default=1000)
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--max-steps", type=int, default=150000)
+    ap.add_argument("--T-range", type=float, nargs=2, default=(170.0, 500.0),
+                    help="training and test range of T (the lattice is unstable at high T on coarse grids)")
+    ap.add_argument("--C-range", type=float, nargs=2, default=(5.0, 20.0))
     ap.add_argument("--min-steps", type=int, default=0)
-    ap.add_argument("--epochs", type=int, default=60)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--width", type=int, default=32)
     ap.add_argument("--modes", type=int, default=12)
     ap.add_argument("--seed", type=int, default=0)
@@ -192,7 +230,9 @@ def main(argv=None):
     t0 = time.perf_counter()
 
     def draws(k):
-        return [(float(rng.uniform(170.0, 800.0)), float(math.exp(rng.uniform(math.log(5.0), math.log(20.0)))))
+        t0_, t1_ = args.T_range
+        c0_, c1_ = args.C_range
+        return [(float(rng.uniform(t0_, t1_)), float(math.exp(rng.uniform(math.log(c0_), math.log(c1_)))))
                 for _ in range(k)]
 
     train_draws, test_draws = draws(args.train), draws(args.test)
@@ -214,10 +254,10 @@ def main(argv=None):
                                           max_steps=args.max_steps, min_steps=args.min_steps,
                                           tag=f"train batch {s // args.batch_boxes}:")
         for p, (T, C), f, r in zip(params, chunk, fields, res):
-            if r.failed:
-                continue
+            if r.failed or not (0.5 < r.nusselt < 5.0):
+                continue     # lattice instability: not a training state
             X.append(features(n, T, C))
-            Y.append(targets_from_fields(p, f, prof_of(p)))
+            Y.append(canonicalize(targets_from_fields(p, f, prof_of(p))))
             gen_steps.append(r.steps)
         print(f"generated {len(X)} states ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
     X = np.stack(X)
@@ -274,6 +314,8 @@ def main(argv=None):
         "generation_s": round(t_gen, 1), "training_s": round(t_train, 1), "evaluation_s": round(t_eval, 1),
         "generation_steps_median": int(np.median(gen_steps)),
         "final_train_loss": losses[-1], "first_epoch_loss": losses[0],
+        "T_range": list(args.T_range), "C_range": list(args.C_range), "epochs": args.epochs,
+        "train_draws": len(train_draws),
         "accepted": len(acc), "acceptance_rate": round(len(acc) / len(rows), 3),
         "steps_ratio_median_accepted": round(float(np.median(speed)), 3) if speed else None,
         "steps_ratio_mean_accepted": round(float(np.mean(speed)), 3) if speed else None,

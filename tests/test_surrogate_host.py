@@ -41,3 +41,26 @@ def test_targets_and_warm_fields_round_trip():
     assert np.allclose(w[6][:, :, 3], (phi[:, :, 2] - phi[:, :, 4]) / (2 * d.dx))
     assert np.allclose(w[8][3], (phi[2] - phi[4]) / (2 * d.dz))
     assert np.array_equal(w[8][0], w[8][1]) and np.array_equal(w[8][-1], w[8][-2])
+
+
+def test_canonicalize_removes_shifts_and_xy_exchange():
+    """Shifted and x/y-exchanged copies of one roll pattern map to the same
+    canonical field."""
+    Z, Y, X = 6, 16, 16
+    z = np.arange(Z)[:, None, None] / (Z - 1)
+    y = np.arange(Y)[None, :, None]
+    x = np.arange(X)[None, None, :]
+    t = np.zeros((5, Z, Y, X))
+    t[2] = np.sin(np.pi * z) * np.cos(2 * np.pi * 2 * x / X + 0.3)
+    t[0] = np.cos(np.pi * z) * np.sin(2 * np.pi * 2 * x / X + 0.3)
+    t[3] = 0.1 * t[2] + 0.05 * np.cos(2 * np.pi * y / Y)
+    ref = S.canonicalize(t)
+    for sx, sy in ((3, 0), (5, 7), (11, 2)):
+        u = np.roll(np.roll(t, sx, axis=3), sy, axis=2)
+        c = S.canonicalize(u)
+        # the pattern part (w, u) is canonical; the y-dependent part of q keeps its own y shift
+        assert np.allclose(c[[0, 2]], ref[[0, 2]], atol=1e-12)
+    sw = t.swapaxes(2, 3).copy()
+    sw[[0, 1]] = sw[[1, 0]]
+    c = S.canonicalize(sw)
+    assert np.allclose(c[[0, 2]], ref[[0, 2]], atol=1e-12)
