@@ -501,11 +501,11 @@ __global__ void __launch_bounds__(256) ffty_kernel(const PoissonJob* __restrict_
 constexpr int ZS_MAXM = 16;
 constexpr int ZS_LOADS = 8;
 
-template <typename T, int P, int MAXM>
-__global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restrict__ jobs, int nmodes, int ny,
+template <typename T, int P, int MAXM, int NT = 256>
+__global__ void __launch_bounds__(NT) zsolve_kernel(const PoissonJob* __restrict__ jobs, int nmodes, int ny,
                                                      int split_mean, double inv_nxy)
 {
-    constexpr int MB = 256 / P;
+    constexpr int MB = NT / P;
     const PoissonJob& J = jobs[blockIdx.y];
     EHD_GLOBAL cplx<T>* __restrict__ spec = gptr((cplx<T>*)J.spec);
     EHD_GLOBAL double* __restrict__ phibar = gptr(J.phibar);
@@ -523,11 +523,11 @@ __global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restric
     const int nm = min(MB, nmodes - m0);
     // the tile load keeps ZS_LOADS loads in flight per thread (a plain loop
     // would wait for each load before its LDS store)
-    for (int w0 = threadIdx.x; w0 < n * MB; w0 += 256 * ZS_LOADS) {
+    for (int w0 = threadIdx.x; w0 < n * MB; w0 += NT * ZS_LOADS) {
         T tx[ZS_LOADS], ty[ZS_LOADS];
 #pragma unroll
         for (int u = 0; u < ZS_LOADS; ++u) {
-            const int w = w0 + u * 256;
+            const int w = w0 + u * NT;
             const int k = w / MB, c = w - k * MB;
             tx[u] = (T)0;
             ty[u] = (T)0;
@@ -539,7 +539,7 @@ __global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restric
         }
 #pragma unroll
         for (int u = 0; u < ZS_LOADS; ++u) {
-            const int w = w0 + u * 256;
+            const int w = w0 + u * NT;
             const int c = w % MB;
             if (w < n * MB && c < nm) tile[w] = cplx<T>{tx[u], ty[u]};
         }
@@ -652,7 +652,7 @@ __global__ void __launch_bounds__(256) zsolve_kernel(const PoissonJob* __restric
         phibar[n + 1] = 0.0;
     }
     __syncthreads();
-    for (int w = threadIdx.x; w < n * MB; w += 256) {
+    for (int w = threadIdx.x; w < n * MB; w += NT) {
         const int k = w / MB, cc = w - k * MB;
         if (cc < nm) stc(spec + (size_t)k * nmodes + m0 + cc, tile[w]);
     }
