@@ -43,6 +43,7 @@ struct ehd_batch {
     void* ditems[2] = {nullptr, nullptr};
     unsigned total_tiles = 0;
     unsigned lbm_grid = 0;
+    unsigned lbm_chunk = 0;   // XCD tile order of the LBM launch (0: linear)
     double lbm_ms = 0.0;
     std::vector<Group> groups;
     std::vector<double> group_ms;
@@ -55,8 +56,8 @@ namespace {
 int batch_lbm(ehd_batch* bt, int cur, bool first, bool ebuf, bool macro)
 {
     if (bt->prec == Prec::F32S) {
-        HIPCHK(launch_lbm_f32s_batch(bt->ditems[cur], (int)bt->boxes.size(), bt->total_tiles, bt->lbm_grid, first,
-                                     ebuf, macro, bt->st));
+        HIPCHK(launch_lbm_f32s_batch(bt->ditems[cur], (int)bt->boxes.size(), bt->total_tiles, bt->lbm_grid, bt->lbm_chunk,
+                                     first, ebuf, macro, bt->st));
     } else {
         for (ehd_box* b : bt->boxes)
             HIPCHK(launch_lbm_f64(b->lp, pop_bufs_at(b, cur), macro_bufs(b), first, ebuf, macro, bt->st));
@@ -155,19 +156,25 @@ int tune(ehd_batch* bt)
         std::vector<unsigned> cands = {0u};
         for (unsigned g : {1024u, 2048u, 4096u, 8192u})
             if (g < T) cands.push_back(g);
+        std::vector<unsigned> chunks = {bt->lbm_chunk};
+        if (bt->lbm_chunk) chunks.push_back(0u);
         double best = 1e30;
-        unsigned bestg = 0;
-        for (unsigned g : cands) {
-            bt->lbm_grid = g;
-            double ms = 0;
-            int rc = time_ms(bt, reps, [&] { return batch_lbm(bt, 0, false, false, false); }, &ms);
-            if (rc) return rc;
-            if (ms < best) {
-                best = ms;
-                bestg = g;
+        unsigned bestg = 0, bestc = bt->lbm_chunk;
+        for (unsigned ch : chunks)
+            for (unsigned g : cands) {
+                bt->lbm_grid = g;
+                bt->lbm_chunk = ch;
+                double ms = 0;
+                int rc = time_ms(bt, reps, [&] { return batch_lbm(bt, 0, false, false, false); }, &ms);
+                if (rc) return rc;
+                if (ms < best) {
+                    best = ms;
+                    bestg = g;
+                    bestc = ch;
+                }
             }
-        }
         bt->lbm_grid = bestg;
+        bt->lbm_chunk = bestc;
         bt->lbm_ms = best;
     }
     bt->group_ms.assign(bt->groups.size(), 0.0);
@@ -295,6 +302,10 @@ int ehd_batch_create(const ehd_params* params, int nboxes, int flags, ehd_batch*
             if (hipMemcpy(bt->ditems[cur], h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess)
                 return bail(set_error(EHD_EHIP, "item table upload failed"));
         }
+        // XCD tile order: superblocks are whole planes when every box has the same plane size
+        bt->lbm_chunk = bt->boxes[0]->lbm_chunk;
+        for (const ehd_box* b : bt->boxes)
+            if (b->lbm_chunk != bt->lbm_chunk) bt->lbm_chunk = 0;
     }
     if (flags & EHD_BATCH_TUNE) {
         int rc = tune(bt);
@@ -432,6 +443,7 @@ int ehd_batch_tuning(ehd_batch* bt, ehd_tuning* out)
     out->lbm_tiles = (int)bt->total_tiles;
     out->lbm_grid = (int)(bt->lbm_grid ? bt->lbm_grid : bt->total_tiles);
     out->lbm_ms = bt->lbm_ms;
+    out->lbm_chunk = (int)bt->lbm_chunk;
     out->ngroups = (int)bt->groups.size();
     for (size_t i = 0; i < bt->groups.size(); ++i) {
         const Group& g = bt->groups[i];

@@ -47,6 +47,7 @@ struct ehd_box {
     ehd::PoissonJob* pjob = nullptr;   // the box's one-entry Poisson job table (device)
     ehd::PoissonJob pjob_h{};          // host copy (batches build their tables from it)
     void* lbm_items[2] = {nullptr, nullptr};   // FP32S: one-item kernel tables per buffer parity
+    unsigned lbm_chunk = 0;              // FP32S LBM XCD tile order (lbm_xcd_chunk)
     int tune_lpb = 0, tune_ylines = 0;   // Poisson lines per block (0: largest that fits)
     long long step = 0;
     double t = 0.0;

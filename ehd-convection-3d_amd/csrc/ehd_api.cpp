@@ -183,7 +183,7 @@ int run_lbm(ehd_box* b, bool macro)
     }
     hipError_t e;
     if (b->prec == Prec::F64) e = launch_lbm_f64(b->lp, pop_bufs(b), macro_bufs(b), b->direct, b->ebuf, macro, b->st);
-    else e = launch_lbm_f32s_batch(b->lbm_items[b->cur], 1, (unsigned)((b->S + 255) / 256), 0, b->direct, b->ebuf,
+    else e = launch_lbm_f32s_batch(b->lbm_items[b->cur], 1, (unsigned)((b->S + 255) / 256), 0, b->lbm_chunk, b->direct, b->ebuf,
                                    macro, b->st);
     if (e != hipSuccess) return set_error(EHD_EHIP, "LBM launch failed: %s", hipGetErrorString(e));
     if (b->prof) {
@@ -513,6 +513,7 @@ int ehd_create(const ehd_params* p, ehd_box** out)
             if (hipMemcpy(b->lbm_items[c], item.data(), isz, hipMemcpyHostToDevice) != hipSuccess)
                 return bail(set_error(EHD_EHIP, "LBM item upload failed"));
         }
+        b->lbm_chunk = lbm_xcd_chunk((unsigned)(p->nx * p->ny));
     }
     *out = b;
     return EHD_OK;

@@ -57,11 +57,16 @@ hipError_t launch_lbm_f64(const LbmParams& P, const PopBufs& pb, const MacroBufs
 // FP32S LBM over one box or a batch of boxes: an opaque item per box (lbm_f32s_batch_item writes
 // lbm_f32s_batch_item_size() bytes), items in a device array sorted by tile0,
 // tile0 = number of 256-node tiles of the boxes before it.  grid = 0 launches
-// one workgroup per tile; a smaller grid makes the workgroups persistent.
+// one workgroup per tile; a smaller grid (a multiple of 8) makes the
+// workgroups persistent.  chunk: XCD tile order (0 = linear), see
+// lbm_xcd_chunk.
 size_t lbm_f32s_batch_item_size();
 void lbm_f32s_batch_item(void* out, const LbmParams& P, const PopBufs& pb, const MacroBufs& mb, unsigned tile0);
-hipError_t launch_lbm_f32s_batch(const void* items, int nitems, unsigned total_tiles, unsigned grid, bool first,
-                                 bool ebuf, bool macro, hipStream_t st);
+hipError_t launch_lbm_f32s_batch(const void* items, int nitems, unsigned total_tiles, unsigned grid, unsigned chunk,
+                                 bool first, bool ebuf, bool macro, hipStream_t st);
+// XCD tile-order chunk for boxes of nxy nodes per plane: one plane per
+// superblock when a plane is a whole number of 8-tile groups, else 0.
+unsigned lbm_xcd_chunk(unsigned nxy);
 
 // init_equilibrium (LBM.cu:98-276) from double macroscopic fields into the
 // pre-collision ("direct") representation of the path.

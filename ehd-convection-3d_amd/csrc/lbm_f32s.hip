@@ -291,10 +291,18 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
 // One launch over one box or a batch of boxes: the boxes' node ranges are
 // cut into 256-node tiles numbered across the batch (box i owns tiles
 // tile0[i] .. tile0[i+1]-1) and the workgroups stride over all tiles (one
-// tile each when the grid covers them, persistent otherwise).  Tiles run in
-// linear order (x fastest, then y, then z): measured faster on MI355X than a
-// remap that gives each XCD a contiguous slab of planes.  A single box is a
-// batch of one, so a box gives the same numbers alone and in a batch.
+// tile each when the grid covers them, persistent otherwise).  A single box
+// is a batch of one, so a box gives the same numbers alone and in a batch.
+//
+// Tile order: the hardware deals workgroup b to XCD b mod 8, and each XCD
+// has its own L2.  With chunk > 0 the tiles are taken in superblocks of
+// 8*chunk (one z-plane of a box when chunk = tiles per plane / 8): XCD j
+// gets the j-th run of chunk consecutive tiles of every superblock, so the
+// potential rows a tile reads around itself (y +- 1, z +- 1) are mostly the
+// rows its own XCD reads as well and come from that L2 instead of HBM.
+// Every superblock still spreads over all XCDs, so the wall planes (more
+// loads per node) do not pile up on one XCD.  chunk = 0 is plain linear
+// order.  The grid is a multiple of 8 whenever it is persistent.
 struct BatchItem {
     LbmParams P;
     F32Consts c;
@@ -305,9 +313,16 @@ struct BatchItem {
 
 template <bool FIRST, bool EBUF>
 __global__ void __launch_bounds__(256) lbm_f32s_kernel(const BatchItem* __restrict__ items, int nitems,
-                                                             unsigned total_tiles, int macro)
+                                                             unsigned total_tiles, unsigned chunk, int macro)
 {
-    for (unsigned tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
+    const unsigned sb = 8u * chunk;
+    const unsigned full = chunk ? (total_tiles / sb) * sb : 0u;
+    for (unsigned v = blockIdx.x; v < total_tiles; v += gridDim.x) {
+        unsigned tile = v;
+        if (v < full) {
+            const unsigned k = v >> 3, j = v & 7u;
+            tile = (k / chunk) * sb + j * chunk + k % chunk;
+        }
         int lo = 0, hi = nitems - 1;   // last item with tile0 <= tile
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -365,18 +380,26 @@ void lbm_f32s_batch_item(void* out, const LbmParams& P, const PopBufs& pb, const
     std::memcpy(out, &it, sizeof(it));
 }
 
-hipError_t launch_lbm_f32s_batch(const void* items, int nitems, unsigned total_tiles, unsigned grid, bool first,
-                                 bool ebuf, bool macro, hipStream_t st)
+hipError_t launch_lbm_f32s_batch(const void* items, int nitems, unsigned total_tiles, unsigned grid, unsigned chunk,
+                                 bool first, bool ebuf, bool macro, hipStream_t st)
 {
     if (nitems < 1 || total_tiles == 0) return hipErrorInvalidValue;
     const BatchItem* it = (const BatchItem*)items;
     const unsigned g = (grid == 0 || grid > total_tiles) ? total_tiles : grid;
+    if (g < total_tiles && g % 8u) return hipErrorInvalidValue;   // the tile order assumes b mod 8 = XCD
     const int m = macro ? 1 : 0;
-    if (first && ebuf) lbm_f32s_kernel<true, true><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
-    else if (first) lbm_f32s_kernel<true, false><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
-    else if (ebuf) lbm_f32s_kernel<false, true><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
-    else lbm_f32s_kernel<false, false><<<g, 256, 0, st>>>(it, nitems, total_tiles, m);
+    if (first && ebuf) lbm_f32s_kernel<true, true><<<g, 256, 0, st>>>(it, nitems, total_tiles, chunk, m);
+    else if (first) lbm_f32s_kernel<true, false><<<g, 256, 0, st>>>(it, nitems, total_tiles, chunk, m);
+    else if (ebuf) lbm_f32s_kernel<false, true><<<g, 256, 0, st>>>(it, nitems, total_tiles, chunk, m);
+    else lbm_f32s_kernel<false, false><<<g, 256, 0, st>>>(it, nitems, total_tiles, chunk, m);
     return hipGetLastError();
+}
+
+unsigned lbm_xcd_chunk(unsigned nxy)
+{
+    if (nxy % 256u) return 0u;
+    const unsigned tpp = nxy / 256u;   // tiles per plane
+    return (tpp % 8u) ? 0u : tpp / 8u;
 }
 
 

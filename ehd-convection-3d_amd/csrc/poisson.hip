@@ -490,20 +490,26 @@ __global__ void __launch_bounds__(256) ffty_kernel(const PoissonJob* __restrict_
 
 // (2) per-mode tridiagonal solve along z, FP64, by the partition method.
 // For each (kx,ky) mode the system is x[k-1] + b x[k] + x[k+1] = dz^2 s[k],
-// k = 0..n-1, x[-1] = x[n] = 0, b = -(2 + k^2 dz^2).  A block holds MB modes
-// (a tile [n][MB] staged in LDS for coalesced access) and P = 256/MB threads
-// per mode, each owning a chunk of at most MAXM (<= ZS_MAXM) consecutive k.  Each thread
-// solves with the chunk's own matrix T (Thomas, in registers)
+// k = 0..n-1, x[-1] = x[n] = 0, b = -(2 + k^2 dz^2).  A block holds MB =
+// NT/P modes and P threads per mode, each owning a chunk of at most MAXM
+// (<= ZS_MAXM) consecutive k; the threads of a wave hold consecutive modes,
+// so every load and store of a chunk row is a contiguous run of modes.  Each
+// thread loads its chunk straight into registers and solves with the
+// chunk's own matrix T (Thomas)
 //   T y = r,  T v = e_first,  (T w = e_last, w = reverse of v for Toeplitz T),
 // so that x = y - x_left v - x_right w.  The 2P interface unknowns (first and
 // last x of every chunk) satisfy a banded system that one thread per mode
-// eliminates in order; the chunks then finish independently.
+// eliminates in order, through per-chunk records in LDS (fields outermost,
+// modes innermost: the threads of a wave hit consecutive banks); the chunks
+// then finish independently and store from registers.  LDS: 6 P MB doubles.
+// (An earlier form staged the [n][MB] tile in LDS for the loads and stores:
+// the same arithmetic, bit-identical results, 31 us against 25 us here on
+// the 256x256x128 FP32 spectrum.)
 constexpr int ZS_MAXM = 16;
-constexpr int ZS_LOADS = 8;
 
 template <typename T, int P, int MAXM, int NT = 256>
 __global__ void __launch_bounds__(NT) zsolve_kernel(const PoissonJob* __restrict__ jobs, int nmodes, int ny,
-                                                     int split_mean, double inv_nxy)
+                                                      int split_mean, double inv_nxy)
 {
     constexpr int MB = NT / P;
     const PoissonJob& J = jobs[blockIdx.y];
@@ -512,64 +518,42 @@ __global__ void __launch_bounds__(NT) zsolve_kernel(const PoissonJob* __restrict
     const int n = J.n;
     const double kxs = J.kxs, kys = J.kys, dz2 = J.dz2, voltage = J.voltage;
     extern __shared__ unsigned char smem_raw[];
-    cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);                 // [n][MB]
-    // per-chunk records, fields outermost and modes innermost so that the
-    // threads of a wave (consecutive modes) hit consecutive banks
-    double* xch = reinterpret_cast<double*>(tile + (size_t)n * MB);       // [6][P][MB]
-    double* res = xch + MB * P * 6;                                        // [4][P][MB]
+    double* xch = reinterpret_cast<double*>(smem_raw);       // [6][P][MB]
     auto X = [&](int f, int q, int cc) -> double& { return xch[(f * P + q) * MB + cc]; };
-    auto R = [&](int f, int q, int cc) -> double& { return res[(f * P + q) * MB + cc]; };
     const int m0 = blockIdx.x * MB;
     const int nm = min(MB, nmodes - m0);
-    // the tile load keeps ZS_LOADS loads in flight per thread (a plain loop
-    // would wait for each load before its LDS store)
-    for (int w0 = threadIdx.x; w0 < n * MB; w0 += NT * ZS_LOADS) {
-        T tx[ZS_LOADS], ty[ZS_LOADS];
-#pragma unroll
-        for (int u = 0; u < ZS_LOADS; ++u) {
-            const int w = w0 + u * NT;
-            const int k = w / MB, c = w - k * MB;
-            tx[u] = (T)0;
-            ty[u] = (T)0;
-            if (w < n * MB && c < nm) {
-                const cplx<T> v = ldc(spec + (size_t)k * nmodes + m0 + c);
-                tx[u] = v.x;
-                ty[u] = v.y;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < ZS_LOADS; ++u) {
-            const int w = w0 + u * NT;
-            const int c = w % MB;
-            if (w < n * MB && c < nm) tile[w] = cplx<T>{tx[u], ty[u]};
-        }
-    }
-    __syncthreads();
     const int c = threadIdx.x % MB, p = threadIdx.x / MB;
     const int m = m0 + c;
     const int Peff = min(P, n);
     const bool active = (c < nm) && (p < Peff);
     const int k0 = (p * n) / Peff, len = ((p + 1) * n) / Peff - k0;
+    T sx[MAXM], sy[MAXM];
+#pragma unroll
+    for (int i = 0; i < MAXM; ++i) {
+        sx[i] = (T)0;
+        sy[i] = (T)0;
+        if (active && i < len) {
+            const cplx<T> v = ldc(spec + (size_t)(k0 + i) * nmodes + m);
+            sx[i] = v.x;
+            sy[i] = v.y;
+        }
+    }
     const int kxi = m / ny, kyi = m - kxi * ny;
     const double kxv = kxi * kxs, kyv = (kyi <= ny / 2 ? kyi : kyi - ny) * kys;
     const double bd = -(2.0 + (kxv * kxv + kyv * kyv) * dz2);
-
     double cp[MAXM], yr[MAXM], yi[MAXM], v[MAXM];
     if (active) {
-        // forward elimination
 #pragma unroll
         for (int i = 0; i < MAXM; ++i) {
             if (i < len) {
                 const double den = (i == 0) ? 1.0 / bd : 1.0 / (bd - cp[i - 1]);
                 cp[i] = den;
-                const cplx<T> sv = tile[(k0 + i) * MB + c];
-                const double rr = dz2 * (double)sv.x, ri = dz2 * (double)sv.y;
+                const double rr = dz2 * (double)sx[i], ri = dz2 * (double)sy[i];
                 yr[i] = (i == 0 ? rr : rr - yr[i - 1]) * den;
                 yi[i] = (i == 0 ? ri : ri - yi[i - 1]) * den;
                 v[i] = (i == 0 ? 1.0 : -v[i - 1]) * den;
             }
         }
-        // back substitution
 #pragma unroll
         for (int i = MAXM - 2; i >= 0; --i) {
             if (i < len - 1) {
@@ -591,14 +575,13 @@ __global__ void __launch_bounds__(NT) zsolve_kernel(const PoissonJob* __restrict
     }
     __syncthreads();
     if (p == 0 && c < nm) {
-        // interface system: x_l(q-1) = al + be * x_f(q) carried along the
-        // chunks; the per-chunk coefficients replace the chunk's exchange
-        // record in LDS (registers stay independent of P)
         double ar = 0.0, ai = 0.0, be = 0.0;
-        for (int qq = 0; qq < Peff; ++qq) {
+#pragma unroll
+        for (int qq = 0; qq < P; ++qq) {
+            if (qq >= Peff) break;
             const double e0 = X(0, qq, c), e1 = X(1, qq, c), e2 = X(2, qq, c), e3 = X(3, qq, c);
             const double vf = X(4, qq, c), vlq = X(5, qq, c);
-            const double wf = vlq, wl = vf;   // persymmetric chunk matrix
+            const double wf = vlq, wl = vf;
             const double den = 1.0 / (1.0 + vf * be);
             const double gr = (e0 - vf * ar) * den;
             const double gi = (e1 - vf * ai) * den;
@@ -610,25 +593,31 @@ __global__ void __launch_bounds__(NT) zsolve_kernel(const PoissonJob* __restrict
             X(0, qq, c) = gr; X(1, qq, c) = gi; X(2, qq, c) = de; X(3, qq, c) = ar; X(4, qq, c) = ai;
             X(5, qq, c) = be;
         }
-        double xfr = 0.0, xfi = 0.0;   // x_f(q+1)
-        for (int qq = Peff - 1; qq >= 0; --qq) {
-            R(2, qq, c) = xfr;     // right neighbour x_f(q+1)
-            R(3, qq, c) = xfi;
-            const double xlr = X(3, qq, c) + X(5, qq, c) * xfr, xli = X(4, qq, c) + X(5, qq, c) * xfi;
+        // back: record q gets (x_l(q-1), x_f(q+1)) in fields 0..3, written
+        // after the record's own coefficients are read
+        double xfr = 0.0, xfi = 0.0;
+#pragma unroll
+        for (int qq = P - 1; qq >= 0; --qq) {
+            if (qq >= Peff) continue;
+            const double gr = X(0, qq, c), gi = X(1, qq, c), de = X(2, qq, c);
+            const double ar = X(3, qq, c), ai = X(4, qq, c), be = X(5, qq, c);
+            const double xlr = ar + be * xfr, xli = ai + be * xfi;
+            X(2, qq, c) = xfr;
+            X(3, qq, c) = xfi;
             if (qq + 1 < Peff) {
-                R(0, qq + 1, c) = xlr;   // left neighbour of chunk q+1
-                R(1, qq + 1, c) = xli;
+                X(0, qq + 1, c) = xlr;
+                X(1, qq + 1, c) = xli;
             }
-            const double nxr = X(0, qq, c) + X(2, qq, c) * xfr, nxi = X(1, qq, c) + X(2, qq, c) * xfi;
+            const double nxr = gr + de * xfr, nxi = gi + de * xfi;
             xfr = nxr;
             xfi = nxi;
         }
-        R(0, 0, c) = 0.0;
-        R(1, 0, c) = 0.0;
+        X(0, 0, c) = 0.0;
+        X(1, 0, c) = 0.0;
     }
     __syncthreads();
     if (active) {
-        const double Lr = R(0, p, c), Li = R(1, p, c), Rr = R(2, p, c), Ri = R(3, p, c);
+        const double Lr = X(0, p, c), Li = X(1, p, c), Rr = X(2, p, c), Ri = X(3, p, c);
         double wn = 0.0;
 #pragma unroll
         for (int i = MAXM - 1; i >= 0; --i) {
@@ -637,12 +626,13 @@ __global__ void __launch_bounds__(NT) zsolve_kernel(const PoissonJob* __restrict
                 wn = wi;
                 const double xr = yr[i] - Lr * v[i] - Rr * wi;
                 const double xi = yi[i] - Li * v[i] - Ri * wi;
+                EHD_GLOBAL cplx<T>* dst = spec + (size_t)(k0 + i) * nmodes + m;
                 if (split_mean && m == 0) {
                     const int z = k0 + i + 1;
                     phibar[z] = voltage * (1.0 - (double)z / (double)(n + 1)) + xr * inv_nxy;
-                    tile[(k0 + i) * MB + c] = {(T)0, (T)0};
+                    stc(dst, cplx<T>{(T)0, (T)0});
                 } else {
-                    tile[(k0 + i) * MB + c] = {(T)xr, (T)xi};
+                    stc(dst, cplx<T>{(T)xr, (T)xi});
                 }
             }
         }
@@ -650,11 +640,6 @@ __global__ void __launch_bounds__(NT) zsolve_kernel(const PoissonJob* __restrict
     if (split_mean && m0 == 0 && threadIdx.x == 0) {
         phibar[0] = voltage;
         phibar[n + 1] = 0.0;
-    }
-    __syncthreads();
-    for (int w = threadIdx.x; w < n * MB; w += NT) {
-        const int k = w / MB, cc = w - k * MB;
-        if (cc < nm) stc(spec + (size_t)k * nmodes + m0 + cc, tile[w]);
     }
 }
 
@@ -782,18 +767,19 @@ int y_lines_per_block(int ny)
     return ((size_t)ny * cs + (size_t)bufs * g * ldl * cs <= kMaxLds) ? g : 0;
 }
 
-// z-solve partition: P (8, 16 or 32) threads per mode with chunks of at most
-// ZS_MAXM and the LDS tile within kMaxLds; fixed by n and the element size so
-// that a box gives the same numbers alone and in a batch.
+// z-solve partition: P (16, 8 or 32) threads per mode with chunks of at most
+// ZS_MAXM; fixed by n so that a box gives the same numbers alone and in a
+// batch.
+// LDS of a z-solve block: six doubles per (chunk, mode), P chunks x 256/P modes
+size_t zsolve_lds(int P) { return (size_t)6 * P * (256 / P) * sizeof(double); }
+
 int zsolve_partition(int n, size_t cs)
 {
-    // 16 threads per mode first (measured fastest at n = 126: 32 us against
-    // 35.6 us for 8 and 62 us for 32 on the 256x256 FP32 spectrum)
-    for (int cand : {16, 8, 32}) {
-        const int mb = 256 / cand;
-        const size_t lds = (size_t)n * mb * cs + (size_t)mb * cand * 10 * sizeof(double);
-        if ((n + cand - 1) / cand <= ZS_MAXM && lds <= kMaxLds) return cand;
-    }
+    (void)cs;
+    // 16 threads per mode first (measured fastest at n = 126: 25 us against
+    // 42 us for 8 and 51 us for 32 on the 256x256 FP32 spectrum)
+    for (int cand : {16, 8, 32})
+        if ((n + cand - 1) / cand <= ZS_MAXM && zsolve_lds(cand) <= kMaxLds) return cand;
     return 0;
 }
 
@@ -826,7 +812,7 @@ hipError_t poisson_group(const PoissonGroup& g, hipStream_t st)
     const int Pz = zsolve_partition(nmax, cs);
     if (Pz == 0 || Pz != g.zs_partition) return hipErrorInvalidValue;
     const int mb = 256 / Pz;
-    const size_t shz = (size_t)nmax * mb * cs + (size_t)mb * Pz * 10 * sizeof(double);
+    const size_t shz = zsolve_lds(Pz);
     const dim3 gz((nmodes + mb - 1) / mb, g.njobs);
     const double inv_nxy = 1.0 / ((double)nx * (double)ny);
     const int sm = g.split_mean;

@@ -74,7 +74,7 @@ class _TuneGroup(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("lbm_grid", C.c_int), ("lbm_tiles", C.c_int), ("lbm_ms", C.c_double), ("ngroups", C.c_int),
-                ("group", _TuneGroup * 16)]
+                ("group", _TuneGroup * 16), ("lbm_chunk", C.c_int)]
 
 
 BATCH_GRAPH, BATCH_TUNE = 1, 2

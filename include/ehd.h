@@ -227,6 +227,7 @@ typedef struct ehd_tuning {
         int nx, ny, nboxes, x_lines, y_lines;   /* lines per workgroup of the x / y FFT passes */
         double ms;                               /* timed Poisson chain of the group, 0 if not tuned */
     } group[16];
+    int lbm_chunk;            /* XCD tile order of the LBM launch: tiles per XCD run (0: linear order) */
 } ehd_tuning;
 
 /* nboxes parameter sets (all of one precision; at most 16 distinct (nx, ny)). */

@@ -334,3 +334,66 @@ def test_fp32s_vs_fp64_short(hd):
     print(f"FP32S vs FP64 after 300 steps (hydro_density={hd}): {errs}")
     assert errs["q"] < 1e-5 and errs["phi"] < 1e-5
     assert max(errs["ux"], errs["uy"], errs["uz"]) < 1e-2
+
+
+def _nine(f):
+    return np.stack([f[k] for k in ("rho", "q", "phi", "ux", "uy", "uz", "ex", "ey", "ez")])
+
+
+def _field_errors(a, r):
+    u_a = np.stack([a["ux"], a["uy"], a["uz"]])[:, 1:-1]
+    u_r = np.stack([r["ux"], r["uy"], r["uz"]])[:, 1:-1]
+    return {"q": rel_l2(a["q"][1:-1], r["q"][1:-1]), "phi": rel_l2(a["phi"], r["phi"]), "u": rel_l2(u_a, u_r)}
+
+
+def _fp32s_vs_fp64(params_of, fields9, nsteps):
+    """Run the same start in both precisions; errors of fields, Nu and current,
+    the FP64 diagnostics and both field sets."""
+    out = {}
+    for prec in (E.FP64, E.FP32S):
+        with E.Box(params_of(prec)) as b:
+            b.init(fields9)
+            b.step(nsteps)
+            out[prec] = (b.read_fields(("q", "phi", "ux", "uy", "uz")), b.diagnostics())
+    (a, da), (r, dr) = out[E.FP32S], out[E.FP64]
+    errs = dict(_field_errors(a, r), Nu=abs(da.nusselt / dr.nusselt - 1), current=abs(da.current / dr.current - 1))
+    return errs, dr, a, r
+
+
+def test_fp32s_vs_fp64_developed_convection():
+    """A developed convecting state (64x64x64, T=300, C=10, grown from noise,
+    Nu ~ 1.8, velocities ~6x the drift velocity): the FP32S path against the
+    FP64 path (which is bit-identical to the oracle's step) over 20,000
+    further steps from the same fields.  Here the populations stray far from
+    the hydrostatic shift, so their float rounding is relative to the
+    convective part itself; the two paths settle to steady states that differ
+    by ~1e-4 in the fields and ~4e-5 in Nu.  Bar: Nu and the current within
+    0.1 %, the fields within 2e-4 rel-L2."""
+    def params_of(prec, init=E.INIT_FIELDS):
+        return E.make_params(64, 64, 64, T=300.0, C_=10.0, lam=0.25, precision=prec, init_mode=init, seed=11,
+                             amp=1e-3, cfl=0.04)
+    with E.Box(params_of(E.FP32S, E.INIT_HYDROSTATIC)) as b:
+        b.init()
+        b.step(120000)
+        start = _nine(b.read_fields())
+    errs, dr, _, _ = _fp32s_vs_fp64(params_of, start, 20000)
+    print(f"developed convection, FP64 Nu {dr.nusselt:.6f}; FP32S vs FP64 after 20000 steps: {errs}")
+    assert dr.nusselt > 1.5
+    assert errs["Nu"] < 1e-3 and errs["current"] < 1e-3, errs
+    assert max(errs["u"], errs["q"], errs["phi"]) < 2e-4, errs
+
+
+def test_fp32s_vs_fp64_256x256x128():
+    """The bench box (256x256x128, T=400, C=10) started from hydrostatic q and
+    phi with roll velocities of 20 % of the drift velocity: 1000 steps in both
+    precisions.  Bar: u, q, phi within 1e-5 rel-L2, Nu and current within 0.1 %."""
+    def params_of(prec, init=E.INIT_FIELDS):
+        return E.make_params(256, 256, 128, T=400.0, C_=10.0, lam=0.25, precision=prec, init_mode=init, seed=2,
+                             amp=0.05, pattern=E.PERTURB_ROLLS)
+    with E.Box(params_of(E.FP64, E.INIT_HYDROSTATIC)) as b:
+        b.init()
+        start = _nine(b.read_fields())
+    errs, dr, _, _ = _fp32s_vs_fp64(params_of, start, 1000)
+    print(f"256x256x128, FP64 Nu {dr.nusselt:.6f}; FP32S vs FP64 after 1000 steps: {errs}")
+    assert errs["u"] < 1e-5 and errs["q"] < 1e-5 and errs["phi"] < 1e-5, errs
+    assert errs["Nu"] < 1e-3 and errs["current"] < 1e-3, errs

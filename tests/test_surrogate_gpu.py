@@ -1,8 +1,11 @@
-"""What the trained surrogate gains over the method without it, on a small
-grid: after training on steady states from the batched solver, the
-network's prediction for held-out (T, C) draws is much closer to their steady
-state than the hydrostatic start the solver otherwise begins from, and the
-warm-started runs reach steady state in fewer steps in total."""
+"""What the trained surrogate gains over the method without it, on 64x64x64
+boxes (C = 10, T between 200 and 300, where the flow settles from noise to
+steady square cells): after training on steady states from the batched
+solver (canonical pattern position), the network's
+prediction for held-out T is much closer to their steady state than the
+hydrostatic start the solver otherwise begins from, the warm-started runs
+end at the cold runs' Nu (the acceptance rule: within 0.5 %), and they
+reach steady state in fewer steps in total."""
 import math
 
 import numpy as np
@@ -13,58 +16,57 @@ from ehd_amd.fno import FNO
 
 pytestmark = pytest.mark.gpu
 
-GRID = (32, 32, 64)
-
-
-def _draws(rng, k):
-    return [(float(rng.uniform(650.0, 750.0)), float(rng.uniform(8.0, 12.0)))
-            for _ in range(k)]
-
-
-def _steady(draws, seed0):
-    params = [S.box_params(GRID, T, C, seed=seed0 + i) for i, (T, C) in enumerate(draws)]
-    res, fields = S.run_batch_to_steady(params, check_every=2000, tol=1e-5, max_steps=150000, tag="test")
-    assert not any(r.failed for r in res)
-    ys = []
-    for p, f in zip(params, fields):
-        ys.append(S.canonicalize(S.targets_from_fields(p, f, S.hydrostatic_profile(p))))
-    return params, res, np.stack(ys)
+GRID = (64, 64, 64)
+TRAIN_T = (200.0, 220.0, 240.0, 260.0, 280.0, 300.0)
+TEST_T = (230.0, 270.0)
+C = 10.0
+RUN = dict(check_every=2000, tol=1e-5, max_steps=300000)
 
 
 def test_warm_start_gain():
-    rng = np.random.default_rng(7)
-    train, test = _draws(rng, 12), _draws(rng, 4)
-    _, _, Ytr = _steady(train, 100)
-    Xtr = np.stack([S.features(GRID, T, C) for T, C in train])
-    net = FNO(*GRID, width=16, modes=(8, 8, 8), layers=4, max_batch=8)
+    # training states and the cold runs of the held-out T, one batch
+    draws = TRAIN_T + TEST_T
+    params = [S.box_params(GRID, T, C, seed=100 + i) for i, T in enumerate(draws)]
+    res, fields = S.run_batch_to_steady(params, tag="cold", **RUN)
+    for T, r in zip(draws, res):
+        print(f"cold T={T:.0f}: {r.steps} steps, Nu {r.nusselt:.5f}, converged {r.converged}, failed {r.failed}")
+    assert all(r.converged for r in res)
+    Y = np.stack([S.canonicalize(S.targets_from_fields(p, f, S.hydrostatic_profile(p)))
+                  for p, f in zip(params, fields)])
+    X = np.stack([S.features(GRID, T, C) for T in draws])
+    ntr = len(TRAIN_T)
+    Xtr, Ytr, Xte, Yte = X[:ntr], Y[:ntr], X[ntr:], Y[ntr:]
+    res_cold = res[ntr:]
+
+    net = FNO(*GRID, width=16, modes=(8, 8, 8), layers=4, max_batch=ntr)
     net.set_params(net.init_params(seed=3))
     losses = []
-    for it in range(300):
-        idx = rng.choice(len(train), size=8, replace=False)
-        lr = 1e-3 * 0.5 * (1 + math.cos(math.pi * it / 300))
-        losses.append(net.train_step(Xtr[idx], Ytr[idx], lr))
+    nit = 400
+    for it in range(nit):
+        lr = 2e-3 * 0.5 * (1 + math.cos(math.pi * it / nit))
+        losses.append(net.train_step(Xtr, Ytr, lr))
     print(f"training loss {np.mean(losses[:10]):.3e} -> {np.mean(losses[-10:]):.3e}")
-    assert np.mean(losses[-10:]) < 0.5 * np.mean(losses[:10])
+    assert np.mean(losses[-10:]) < 0.2 * np.mean(losses[:10])
 
-    params, res_cold, Yte = _steady(test, 900)
-    pred = net.forward(np.stack([S.features(GRID, T, C) for T, C in test]))
-    # field error (canonical phase) of the network vs the hydrostatic start
-    # (zero velocity, hydrostatic q and phi)
-    err_net = np.linalg.norm(pred - Yte) / np.linalg.norm(Yte)
-    err_hs = 1.0
-    print(f"held-out rel-L2 to the steady state: network {err_net:.3f}, hydrostatic start {err_hs:.3f}")
-    assert err_net < 0.5 * err_hs
+    pred = net.forward(Xte)
+    # distance to the steady state (canonical phase): network vs the hydrostatic
+    # start (zero velocity, hydrostatic q and phi, i.e. all targets zero)
+    err_net = float(np.linalg.norm(pred - Yte) / np.linalg.norm(Yte))
+    print(f"held-out rel-L2 to the steady state: network {err_net:.3f}, hydrostatic start 1.000")
+    assert err_net < 0.5
 
-    # steps to steady: warm starts from the prediction (plus the same noise) vs the cold runs above
-    warm_params, fields = [], []
-    for i, ((T, C), y) in enumerate(zip(test, pred)):
+    warm_params, warm_fields = [], []
+    for i, (T, y) in enumerate(zip(TEST_T, pred)):
         pw = S.box_params(GRID, T, C, seed=900 + i, init=S.INIT_FIELDS)
         warm_params.append(pw)
-        fields.append(S.fields9_from_prediction(pw, y, S.hydrostatic_profile(pw), noise_seed=900 + i, amp=1e-3))
-    res_warm, _ = S.run_batch_to_steady(warm_params, fields, check_every=2000, tol=1e-5, max_steps=150000, tag="warm")
+        warm_fields.append(S.fields9_from_prediction(pw, y, S.hydrostatic_profile(pw), noise_seed=900 + i, amp=1e-3))
+    res_warm, _ = S.run_batch_to_steady(warm_params, warm_fields, tag="warm", **RUN)
     cold = sum(r.steps for r in res_cold)
     warm = sum(r.steps for r in res_warm)
-    for (T, C), c, w in zip(test, res_cold, res_warm):
-        print(f"T={T:.0f} C={C:.1f}: cold {c.steps} steps Nu {c.nusselt:.4f}; warm {w.steps} steps Nu {w.nusselt:.4f}")
-    print(f"total steps: cold {cold}, warm {warm}")
+    for T, c, w in zip(TEST_T, res_cold, res_warm):
+        print(f"T={T:.0f}: cold {c.steps} steps Nu {c.nusselt:.5f}; warm {w.steps} steps Nu {w.nusselt:.5f}")
+    print(f"total steps to steady: cold {cold}, warm {warm} (ratio {cold / warm:.2f})")
+    assert all(w.converged for w in res_warm)
+    for c, w in zip(res_cold, res_warm):
+        assert abs(w.nusselt / c.nusselt - 1) < 5e-3
     assert warm < cold

@@ -44,23 +44,56 @@ def test_targets_and_warm_fields_round_trip():
 
 
 def test_canonicalize_removes_shifts_and_xy_exchange():
-    """Shifted and x/y-exchanged copies of one roll pattern map to the same
-    canonical field."""
+    """Shifted (by whole and fractional nodes) and x/y-exchanged copies of one
+    roll pattern map to the same canonical field, with the roll's upflow at
+    x = 0."""
     Z, Y, X = 6, 16, 16
     z = np.arange(Z)[:, None, None] / (Z - 1)
-    y = np.arange(Y)[None, :, None]
     x = np.arange(X)[None, None, :]
-    t = np.zeros((5, Z, Y, X))
-    t[2] = np.sin(np.pi * z) * np.cos(2 * np.pi * 2 * x / X + 0.3)
-    t[0] = np.cos(np.pi * z) * np.sin(2 * np.pi * 2 * x / X + 0.3)
-    t[3] = 0.1 * t[2] + 0.05 * np.cos(2 * np.pi * y / Y)
-    ref = S.canonicalize(t)
-    for sx, sy in ((3, 0), (5, 7), (11, 2)):
-        u = np.roll(np.roll(t, sx, axis=3), sy, axis=2)
-        c = S.canonicalize(u)
-        # the pattern part (w, u) is canonical; the y-dependent part of q keeps its own y shift
-        assert np.allclose(c[[0, 2]], ref[[0, 2]], atol=1e-12)
-    sw = t.swapaxes(2, 3).copy()
+    y = np.arange(Y)[None, :, None]
+
+    def rolls(x0):
+        t = np.zeros((5, Z, Y, X))
+        a = 2 * np.pi * 2 * (x - x0) / X
+        t[2] = np.sin(np.pi * z) * np.cos(a) + 0.1 * np.sin(np.pi * z) * np.cos(2 * a)
+        t[0] = np.cos(np.pi * z) * np.sin(a)
+        t[3] = 0.1 * t[2]
+        t[4] = 0.01 * np.sin(np.pi * z) * np.cos(a) + 0 * y
+        return t
+    ref = S.canonicalize(rolls(0.0))
+    assert np.allclose(ref, rolls(0.0), atol=1e-12)
+    for x0 in (3.0, 5.25, 11.7, -0.4):
+        assert np.allclose(S.canonicalize(rolls(x0)), ref, atol=1e-12)
+    sw = rolls(2.3).swapaxes(2, 3).copy()
     sw[[0, 1]] = sw[[1, 0]]
-    c = S.canonicalize(sw)
-    assert np.allclose(c[[0, 2]], ref[[0, 2]], atol=1e-12)
+    assert np.allclose(S.canonicalize(sw), ref, atol=1e-12)
+
+
+def test_canonicalize_cells_both_axes():
+    """Square cells (sum and product forms) and hexagon-like cells shifted
+    along x and y by any amounts (the position a noise start picks) map to
+    one canonical field."""
+    Z, Y, X = 5, 12, 12
+    z = np.arange(Z)[:, None, None] / (Z - 1)
+    x = np.arange(X)[None, None, :]
+    y = np.arange(Y)[None, :, None]
+
+    def cells(kind, x0, y0):
+        ax, ay = 2 * np.pi * (x - x0) / X, 2 * np.pi * (y - y0) / Y
+        t = np.zeros((5, Z, Y, X))
+        if kind == "sum":
+            w = np.cos(ax) + np.cos(ay)
+        elif kind == "product":
+            w = np.cos(ax) * np.cos(ay)
+        else:
+            w = np.cos(2 * ax) + 0.6 * np.cos(ax + ay) + 0.3 * np.cos(ax - 2 * ay)
+        t[2] = np.sin(np.pi * z) * w
+        t[0] = 0.5 * np.cos(np.pi * z) * (np.roll(w, -1, axis=2) - np.roll(w, 1, axis=2))
+        t[1] = 0.5 * np.cos(np.pi * z) * (np.roll(w, -1, axis=1) - np.roll(w, 1, axis=1))
+        t[3] = 0.2 * t[2] + 0.05 * np.sin(np.pi * z)
+        t[4] = 0.01 * t[2]
+        return t
+    for kind in ("sum", "product", "mixed"):
+        ref = S.canonicalize(cells(kind, 0.3, -1.1))
+        for x0, y0 in ((1.0, 7.0), (4.6, 2.2), (10.1, 11.9)):
+            assert np.allclose(S.canonicalize(cells(kind, x0, y0)), ref, atol=1e-10), kind
