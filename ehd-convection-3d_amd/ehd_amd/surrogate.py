@@ -189,6 +189,11 @@ def fields9_from_prediction(p, y, prof, noise_seed=None, amp=0.0):
     f = np.zeros((9, nz, ny, nx))
     f[0] = p.rho0
     f[1] = qz[:, None, None] + Q * y[3]
+    # The injection wall takes the injected charge, not a prediction: its
+    # charge populations are rebuilt from their own previous values every step
+    # (LBM.cu:1281-1353), so q(z=0) follows q' = 2 q_inj - q and a start away
+    # from q_inj oscillates with period 2 for ever and shifts the steady state.
+    f[1][0] = qz[0]
     f[2] = phz[:, None, None] + V * y[4]
     f[2][0] = p.voltage
     f[2][-1] = 0.0

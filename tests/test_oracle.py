@@ -174,6 +174,31 @@ def test_injection_bc():
     assert np.array_equal(b[O.H0][1:], h0[1:])
 
 
+def test_injection_wall_charge_has_period_two_memory():
+    """A consequence of the injection rule (LBM.cu:1281-1353) and of a
+    collision that conserves charge: the z=0 charge populations are rebuilt
+    from their own post-collision values, so the wall charge follows
+    q' = 2 q_inj - q and never forgets its start.  From the injected value it
+    stays put; from q_inj + a it alternates q_inj - a, q_inj + a, ... (warm
+    starts must therefore begin the wall at q_inj)."""
+    ob = O.Box(O.make_params(4, 4, 8, T=200.0, M=10.0, C_=10.0, Fe=4000.0, lam=0.25))
+    ob.init_hydrostatic(1, 1e-3, 0)
+    qinj = ob[O.Q][0].copy()
+    assert np.allclose(qinj, ob.p.charge0, rtol=1e-14)
+    ob.init_equilibrium()
+    ob.step(3)
+    assert np.allclose(ob[O.Q][0], qinj, rtol=1e-12)
+    ob.init_hydrostatic(1, 1e-3, 0)
+    ob[O.Q][0] += 0.3 * qinj
+    ob.init_equilibrium()
+    seq = []
+    for _ in range(6):
+        ob.step(1)
+        seq.append(ob[O.Q][0].copy())
+    for k in range(6):
+        assert np.allclose(seq[k], qinj * (0.7 if k % 2 else 1.3), rtol=1e-12)
+
+
 # --------------------------------------------------------------------------- collision
 
 def _xi(p):

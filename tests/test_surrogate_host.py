@@ -35,6 +35,7 @@ def test_targets_and_warm_fields_round_trip():
     assert np.allclose(w[3][1:-1], f["ux"][1:-1], rtol=1e-6, atol=1e-9 * U)
     assert np.allclose(w[5][1:-1], f["uz"][1:-1], rtol=1e-6, atol=1e-9 * U)
     assert np.allclose(w[1][1:-1], f["q"][1:-1], rtol=1e-6)
+    assert np.all(w[1][0] == qz[0])          # the injection wall starts at the injected charge
     assert np.allclose(w[2], f["phi"], rtol=1e-6, atol=1e-6 * V)
     assert np.all(w[3:6][:, 0] == 0) and np.all(w[3:6][:, -1] == 0) and np.all(w[0] == p.rho0)
     phi = w[2]
