@@ -53,11 +53,12 @@ struct ehd_batch {
 
 namespace {
 
-int batch_lbm(ehd_batch* bt, int cur, bool first, bool ebuf, bool macro)
+// titems: TARGET item tables per parity (consistent start), or null
+int batch_lbm(ehd_batch* bt, int cur, bool first, bool ebuf, bool macro, void* const* titems = nullptr)
 {
     if (bt->prec == Prec::F32S) {
-        HIPCHK(launch_lbm_f32s_batch(bt->ditems[cur], (int)bt->boxes.size(), bt->total_tiles, bt->lbm_grid, bt->lbm_chunk,
-                                     first, ebuf, macro, bt->st));
+        HIPCHK(launch_lbm_f32s_batch(titems ? titems[cur] : bt->ditems[cur], (int)bt->boxes.size(), bt->total_tiles,
+                                     bt->lbm_grid, bt->lbm_chunk, first, ebuf, macro, bt->st, titems != nullptr));
     } else {
         for (ehd_box* b : bt->boxes)
             HIPCHK(launch_lbm_f64(b->lp, pop_bufs_at(b, cur), macro_bufs(b), first, ebuf, macro, bt->st));
@@ -329,6 +330,84 @@ int ehd_batch_init(ehd_batch* bt, const double* const* fields9)
         int rc = ehd_init(bt->boxes[i], fields9 ? fields9[i] : nullptr);
         if (rc) return rc;
     }
+    return EHD_OK;
+}
+
+int ehd_batch_init_consistent(ehd_batch* bt, const double* const* fields9, int iterations)
+{
+    if (!bt || !fields9) return set_error(EHD_EINVAL, "null argument");
+    if (bt->prec != Prec::F32S) return set_error(EHD_EINVAL, "the consistent start is implemented for FP32S boxes");
+    if (iterations < 0) return set_error(EHD_EINVAL, "iterations < 0");
+    const size_t n = bt->boxes.size();
+    for (size_t i = 0; i < n; ++i)
+        if (fields9[i] && bt->boxes[i]->prm.init_mode != EHD_INIT_FIELDS)
+            return set_error(EHD_EINVAL, "box %zu: the consistent start needs EHD_INIT_FIELDS", i);
+    int rc = ehd_batch_init(bt, fields9);
+    if (rc || iterations == 0) return rc;
+    // targets of the boxes that have fields; TARGET item tables of the whole batch
+    std::vector<double*> tg(n, nullptr);
+    void* titems[2] = {nullptr, nullptr};
+    struct Release {
+        std::vector<double*>& tg;
+        void** it;
+        hipStream_t st;
+        ~Release()
+        {
+            (void)hipStreamSynchronize(st);
+            for (double* p : tg)
+                if (p) (void)hipFree(p);
+            for (int c = 0; c < 2; ++c)
+                if (it[c]) (void)hipFree(it[c]);
+        }
+    } release{tg, titems, bt->st};
+    const int src[4] = {3, 4, 5, 1};   // fields9 -> ux, uy, uz, q
+    for (size_t i = 0; i < n; ++i) {
+        if (!fields9[i]) continue;
+        const size_t S = bt->boxes[i]->S;
+        if (hipMalloc(&tg[i], 4 * S * sizeof(double)) != hipSuccess) {
+            tg[i] = nullptr;
+            return set_error(EHD_ENOMEM, "target allocation failed");
+        }
+        for (int k = 0; k < 4; ++k)
+            HIPCHK(hipMemcpyAsync(tg[i] + k * S, fields9[i] + src[k] * S, S * sizeof(double), hipMemcpyHostToDevice,
+                                  bt->st));
+    }
+    const size_t isz = lbm_f32s_batch_item_size();
+    for (int cur = 0; cur < 2; ++cur) {
+        std::vector<unsigned char> h(isz * n);
+        unsigned tile0 = 0;
+        for (size_t i = 0; i < n; ++i) {
+            ehd_box* b = bt->boxes[i];
+            lbm_f32s_batch_item(h.data() + i * isz, b->lp, pop_bufs_at(b, cur), macro_bufs(b), tile0, tg[i]);
+            tile0 += (unsigned)((b->S + 255) / 256);
+        }
+        if (hipMalloc(&titems[cur], h.size()) != hipSuccess) {
+            titems[cur] = nullptr;
+            return set_error(EHD_ENOMEM, "item table allocation failed");
+        }
+        HIPCHK(hipMemcpyAsync(titems[cur], h.data(), h.size(), hipMemcpyHostToDevice, bt->st));
+        HIPCHK(hipStreamSynchronize(bt->st));
+    }
+    for (ehd_box* b : bt->boxes) HIPCHK(hipStreamSynchronize(b->st));
+    int cur = bt->boxes[0]->cur;
+    bool direct = bt->boxes[0]->direct, ebuf = bt->boxes[0]->ebuf;
+    for (int i = 0; i < iterations; ++i) {
+        if ((rc = batch_lbm(bt, cur, direct, ebuf, i == iterations - 1, titems))) return rc;
+        if ((rc = batch_poisson(bt))) return rc;
+        cur = 1 - cur;
+        direct = false;
+        ebuf = false;
+        for (ehd_box* b : bt->boxes) {
+            b->step += 1;
+            b->t = b->t + b->der.dt;
+        }
+    }
+    for (ehd_box* b : bt->boxes) {
+        b->cur = cur;
+        b->direct = direct;
+        b->ebuf = ebuf;
+    }
+    HIPCHK(hipStreamSynchronize(bt->st));
     return EHD_OK;
 }
 

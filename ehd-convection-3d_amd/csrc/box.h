@@ -64,8 +64,21 @@ namespace ehd {
 PopBufs pop_bufs(ehd_box* b);
 MacroBufs macro_bufs(ehd_box* b);
 // one LBM launch on the box's stream (macro: also write rho and u), flips the buffers
-int run_lbm(ehd_box* b, bool macro);
+// target_items: FP32S one-item tables (per buffer parity) carrying
+// consistent-start targets; null for an ordinary step.
+int run_lbm(ehd_box* b, bool macro, void* const* target_items = nullptr);
 // one Poisson solve on the box's stream
 int run_poisson(ehd_box* b);
+
+// Consistent-start targets of one FP32S box: [ux, uy, uz, q] from a
+// fields9 array on the device and the two TARGET item tables; released on
+// destruction (after the box's stream has drained).
+struct ConsistentTargets {
+    double* tgt = nullptr;
+    void* items[2] = {nullptr, nullptr};
+    hipStream_t st = nullptr;
+    int make(ehd_box* b, const double* fields9);
+    ~ConsistentTargets();
+};
 PoissonGroup poisson_group_of(const ehd_box* b);
 }  // namespace ehd

@@ -172,7 +172,7 @@ MacroBufs macro_bufs(ehd_box* b)
     return MacroBufs{b->rho, b->ux, b->uy, b->uz, b->q, b->phi, b->phibar, b->ex, b->ey, b->ez};
 }
 
-int run_lbm(ehd_box* b, bool macro)
+int run_lbm(ehd_box* b, bool macro, void* const* target_items)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (b->prof) {
@@ -183,8 +183,9 @@ int run_lbm(ehd_box* b, bool macro)
     }
     hipError_t e;
     if (b->prec == Prec::F64) e = launch_lbm_f64(b->lp, pop_bufs(b), macro_bufs(b), b->direct, b->ebuf, macro, b->st);
-    else e = launch_lbm_f32s_batch(b->lbm_items[b->cur], 1, (unsigned)((b->S + 255) / 256), 0, b->lbm_chunk, b->direct, b->ebuf,
-                                   macro, b->st);
+    else e = launch_lbm_f32s_batch(target_items ? target_items[b->cur] : b->lbm_items[b->cur], 1,
+                                   (unsigned)((b->S + 255) / 256), 0, b->lbm_chunk, b->direct, b->ebuf, macro, b->st,
+                                   target_items != nullptr);
     if (e != hipSuccess) return set_error(EHD_EHIP, "LBM launch failed: %s", hipGetErrorString(e));
     if (b->prof) {
         HIPCHK(hipEventRecord(e1, b->st));
@@ -229,6 +230,37 @@ int run_poisson(ehd_box* b)
     }
     b->ebuf = false;
     return EHD_OK;
+}
+
+int ConsistentTargets::make(ehd_box* b, const double* fields9)
+{
+    const size_t S = b->S;
+    st = b->st;
+    int rc = alloc((void**)&tgt, 4 * S * sizeof(double));
+    if (rc) return rc;
+    // fields9: rho, q, phi, ux, uy, uz, ex, ey, ez; targets: ux, uy, uz, q
+    const int src[4] = {3, 4, 5, 1};
+    for (int k = 0; k < 4; ++k)
+        HIPCHK(hipMemcpyAsync(tgt + k * S, fields9 + src[k] * S, S * sizeof(double), hipMemcpyHostToDevice, st));
+    const size_t isz = lbm_f32s_batch_item_size();
+    std::vector<unsigned char> item(isz);
+    for (int c = 0; c < 2; ++c) {
+        const int A = c, B = 1 - c;
+        const PopBufs pb{b->f[A], b->f0[A], b->h[A], b->h0[A], b->f[B], b->f0[B], b->h[B], b->h0[B]};
+        lbm_f32s_batch_item(item.data(), b->lp, pb, macro_bufs(b), 0, tgt);
+        if ((rc = alloc(&items[c], isz))) return rc;
+        HIPCHK(hipMemcpyAsync(items[c], item.data(), isz, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return EHD_OK;
+}
+
+ConsistentTargets::~ConsistentTargets()
+{
+    if (st) (void)hipStreamSynchronize(st);
+    for (void* p : items)
+        if (p) (void)hipFree(p);
+    if (tgt) (void)hipFree(tgt);
 }
 
 }  // namespace ehd
@@ -633,6 +665,28 @@ int ehd_step(ehd_box* b, int nsteps)
         int rc = resolve_events(b);
         if (rc) return rc;
     }
+    return EHD_OK;
+}
+
+int ehd_init_consistent(ehd_box* b, const double* fields9, int iterations)
+{
+    if (!b || !fields9) return set_error(EHD_EINVAL, "null argument");
+    if (b->prec != Prec::F32S) return set_error(EHD_EINVAL, "the consistent start is implemented for FP32S boxes");
+    if (b->prm.init_mode != EHD_INIT_FIELDS) return set_error(EHD_EINVAL, "the consistent start needs EHD_INIT_FIELDS");
+    if (iterations < 0) return set_error(EHD_EINVAL, "iterations < 0");
+    int rc = ehd_init(b, fields9);
+    if (rc || iterations == 0) return rc;
+    const size_t S = b->S;
+    ConsistentTargets ct;
+    if ((rc = ct.make(b, fields9))) return rc;
+    for (int i = 0; i < iterations; ++i) {
+        if ((rc = run_lbm(b, i == iterations - 1, ct.items))) return rc;
+        if ((rc = run_poisson(b))) return rc;
+        b->step += 1;
+        b->t = b->t + b->der.dt;
+    }
+    (void)S;
+    HIPCHK(hipStreamSynchronize(b->st));
     return EHD_OK;
 }
 

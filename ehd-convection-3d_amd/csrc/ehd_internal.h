@@ -61,9 +61,12 @@ hipError_t launch_lbm_f64(const LbmParams& P, const PopBufs& pb, const MacroBufs
 // workgroups persistent.  chunk: XCD tile order (0 = linear), see
 // lbm_xcd_chunk.
 size_t lbm_f32s_batch_item_size();
-void lbm_f32s_batch_item(void* out, const LbmParams& P, const PopBufs& pb, const MacroBufs& mb, unsigned tile0);
+// target: consistent-start target fields [ux, uy, uz, q] (4 S doubles) of the
+// box, used only by launches with target = true (null: the box steps normally).
+void lbm_f32s_batch_item(void* out, const LbmParams& P, const PopBufs& pb, const MacroBufs& mb, unsigned tile0,
+                         const double* target = nullptr);
 hipError_t launch_lbm_f32s_batch(const void* items, int nitems, unsigned total_tiles, unsigned grid, unsigned chunk,
-                                 bool first, bool ebuf, bool macro, hipStream_t st);
+                                 bool first, bool ebuf, bool macro, hipStream_t st, bool target = false);
 // XCD tile-order chunk for boxes of nxy nodes per plane: one plane per
 // superblock when a plane is a whole number of 8-tile groups, else 0.
 unsigned lbm_xcd_chunk(unsigned nxy);

@@ -71,9 +71,17 @@ __device__ __forceinline__ float ldnt(cgfloat* p) { return __builtin_nontemporal
 __device__ __forceinline__ void stnt(float v, gfloat* p) { __builtin_nontemporal_store(v, p); }
 
 // One node s of one box: pull, collide, boundary and injection.
-template <bool FIRST, bool EBUF>
+//
+// TARGET (consistent start, after Mei et al., J. Comput. Phys. 2006): when
+// the box carries target fields tg = [ux, uy, uz, q] (4 S doubles), every
+// interior node collides towards the equilibria of the target velocity and
+// charge instead of its own (its density stays free), the force uses the
+// target charge, and the target charge is what the Poisson solve sees.
+// Iterated, this builds the non-equilibrium parts of the populations that
+// belong to the target state; the equilibrium start alone lacks them.
+template <bool FIRST, bool EBUF, bool TARGET>
 __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Consts& c, const PopBufs& pb,
-                                              const MacroBufs& mb, int macro, unsigned s)
+                                              const MacroBufs& mb, const double* tgp, int macro, unsigned s)
 {
     const unsigned nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy, S = P.S;
     const unsigned x = s % nx, t = s / nx, y = t % ny, z = t / ny;
@@ -143,6 +151,16 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
     }
     const float rho = rs + drho;
     const float charge = qs + dq;
+    // charge of the equilibrium and of the force (the node's own unless TARGET)
+    float qe = charge, dqe = dq;
+    bool tnode = false;
+    if constexpr (TARGET) {
+        if (tgp && z > 0 && z < nz - 1) {
+            tnode = true;
+            qe = (float)gd(tgp)[3 * (size_t)S + s];
+            dqe = qe - qs;
+        }
+    }
     float j[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -154,7 +172,7 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
         }
         j[a] = pp - nn;
     }
-    const float Fx = charge * Ex + c.exf, Fy = charge * Ey, Fz = charge * Ez;
+    const float Fx = qe * Ex + c.exf, Fy = qe * Ey, Fz = qe * Ez;
     const float hdt = 0.5f * c.dt;
     const float rinv = 1.0f / rho;
     float ux = rinv * (j[0] * c.cflinv + Fx * hdt);
@@ -208,13 +226,22 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
         uz = -rinv * (jm[2] * c.cflinv + (qm * Ezm) * hdt);
     }
 
+    if constexpr (TARGET) {
+        if (tnode) {
+            cgdouble* tg = gd(tgp);
+            ux = (float)tg[s];
+            uy = (float)tg[(size_t)S + s];
+            uz = (float)tg[2 * (size_t)S + s];
+        }
+    }
+
     if (macro) {
         gw(mb.rho)[s] = rho;
         gw(mb.ux)[s] = ux;
         gw(mb.uy)[s] = uy;
         gw(mb.uz)[s] = uz;
     }
-    gw(mb.q)[s] = charge;
+    gw(mb.q)[s] = qe;
 
     const float ucx = ux + c.K * Ex, ucy = uy + c.K * Ey, ucz = uz + c.K * Ez;
     const float U2 = (ux * ux + uy * uy + uz * uz) * c.inv2cs2;
@@ -230,7 +257,7 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
     // rest populations
     {
         const float geq0 = c.w[0] * (drho - rho * U2);
-        const float keq0 = c.w[0] * (dq - charge * Uc2);
+        const float keq0 = c.w[0] * (dqe - qe * Uc2);
         const float F0 = -c.w[0] * uF;
         const float g0n = g[0] - c.omp * (g[0] - geq0) + c.dt * c.sp * F0;
         const float k0n = k[0] - c.ocp * (k[0] - keq0);
@@ -250,8 +277,8 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
         const float tt = c.A * cu, tc = c.A * cc;
         const float geqp = w * (drho + rho * (0.5f * tt * tt - U2));
         const float geqm = w * rho * tt;
-        const float keqp = w * (dq + charge * (0.5f * tc * tc - Uc2));
-        const float keqm = w * charge * tc;
+        const float keqp = w * (dqe + qe * (0.5f * tc * tc - Uc2));
+        const float keqm = w * qe * tc;
         const float Fp = w * (c.B * cu * cF - uF);
         const float Fm = w * (c.A * cF);
         const float gp = 0.5f * (g[d] + g[e]), gmn = 0.5f * (g[d] - g[e]);
@@ -308,10 +335,11 @@ struct BatchItem {
     F32Consts c;
     PopBufs pb;
     MacroBufs mb;
+    const double* target;   // consistent-start target fields, or null
     unsigned tile0, ntiles;
 };
 
-template <bool FIRST, bool EBUF>
+template <bool FIRST, bool EBUF, bool TARGET>
 __global__ void __launch_bounds__(256) lbm_f32s_kernel(const BatchItem* __restrict__ items, int nitems,
                                                              unsigned total_tiles, unsigned chunk, int macro)
 {
@@ -331,7 +359,8 @@ __global__ void __launch_bounds__(256) lbm_f32s_kernel(const BatchItem* __restri
         }
         const BatchItem& it = items[lo];
         const unsigned s = (tile - it.tile0) * 256u + threadIdx.x;
-        if (s < it.P.S) lbm_f32s_node<FIRST, EBUF>(it.P, it.c, it.pb, it.mb, macro, s);
+        if (s < it.P.S) lbm_f32s_node<FIRST, EBUF, TARGET>(it.P, it.c, it.pb, it.mb, TARGET ? it.target : nullptr,
+                                                           macro, s);
     }
 }
 
@@ -368,30 +397,40 @@ F32Consts make_f32consts(const LbmParams& P)
 
 size_t lbm_f32s_batch_item_size() { return sizeof(BatchItem); }
 
-void lbm_f32s_batch_item(void* out, const LbmParams& P, const PopBufs& pb, const MacroBufs& mb, unsigned tile0)
+void lbm_f32s_batch_item(void* out, const LbmParams& P, const PopBufs& pb, const MacroBufs& mb, unsigned tile0,
+                         const double* target)
 {
     BatchItem it{};
     it.P = P;
     it.c = make_f32consts(P);
     it.pb = pb;
     it.mb = mb;
+    it.target = target;
     it.tile0 = tile0;
     it.ntiles = (P.S + 255) / 256;
     std::memcpy(out, &it, sizeof(it));
 }
 
+template <bool TARGET>
+static void launch_variant(const BatchItem* it, int nitems, unsigned total_tiles, unsigned g, unsigned chunk, bool first,
+                    bool ebuf, int m, hipStream_t st)
+{
+    if (first && ebuf) lbm_f32s_kernel<true, true, TARGET><<<g, 256, 0, st>>>(it, nitems, total_tiles, chunk, m);
+    else if (first) lbm_f32s_kernel<true, false, TARGET><<<g, 256, 0, st>>>(it, nitems, total_tiles, chunk, m);
+    else if (ebuf) lbm_f32s_kernel<false, true, TARGET><<<g, 256, 0, st>>>(it, nitems, total_tiles, chunk, m);
+    else lbm_f32s_kernel<false, false, TARGET><<<g, 256, 0, st>>>(it, nitems, total_tiles, chunk, m);
+}
+
 hipError_t launch_lbm_f32s_batch(const void* items, int nitems, unsigned total_tiles, unsigned grid, unsigned chunk,
-                                 bool first, bool ebuf, bool macro, hipStream_t st)
+                                 bool first, bool ebuf, bool macro, hipStream_t st, bool target)
 {
     if (nitems < 1 || total_tiles == 0) return hipErrorInvalidValue;
     const BatchItem* it = (const BatchItem*)items;
     const unsigned g = (grid == 0 || grid > total_tiles) ? total_tiles : grid;
     if (g < total_tiles && g % 8u) return hipErrorInvalidValue;   // the tile order assumes b mod 8 = XCD
     const int m = macro ? 1 : 0;
-    if (first && ebuf) lbm_f32s_kernel<true, true><<<g, 256, 0, st>>>(it, nitems, total_tiles, chunk, m);
-    else if (first) lbm_f32s_kernel<true, false><<<g, 256, 0, st>>>(it, nitems, total_tiles, chunk, m);
-    else if (ebuf) lbm_f32s_kernel<false, true><<<g, 256, 0, st>>>(it, nitems, total_tiles, chunk, m);
-    else lbm_f32s_kernel<false, false><<<g, 256, 0, st>>>(it, nitems, total_tiles, chunk, m);
+    if (target) launch_variant<true>(it, nitems, total_tiles, g, chunk, first, ebuf, m, st);
+    else launch_variant<false>(it, nitems, total_tiles, g, chunk, first, ebuf, m, st);
     return hipGetLastError();
 }
 

@@ -86,6 +86,7 @@ EXPORTS = (
     "ehd_set_nusselt_reference", "ehd_profile_enable", "ehd_profile_read", "ehd_profile_reset",
     "ehd_sync", "ehd_last_error", "ehd_abi_version", "ehd_save_state", "ehd_load_state",
     "ehd_hydrostatic_current", "ehd_run_to_steady", "ehd_hydrostatic_reference",
+    "ehd_init_consistent", "ehd_batch_init_consistent",
     "ehd_batch_create", "ehd_batch_destroy", "ehd_batch_init", "ehd_batch_step", "ehd_batch_diagnostics",
     "ehd_batch_run_to_steady", "ehd_batch_box", "ehd_batch_tuning",
     # host-only file layer, include/ehd_io.h
@@ -112,6 +113,7 @@ def lib() -> C.CDLL:
     L.ehd_destroy.argtypes = [vp]
     L.ehd_destroy.restype = None
     L.ehd_init.argtypes = [vp, dp]
+    L.ehd_init_consistent.argtypes = [vp, dp, C.c_int]
     L.ehd_step.argtypes = [vp, C.c_int]
     L.ehd_lbm_step.argtypes = [vp]
     L.ehd_poisson.argtypes = [vp]
@@ -142,6 +144,7 @@ def lib() -> C.CDLL:
     L.ehd_batch_destroy.argtypes = [vp]
     L.ehd_batch_destroy.restype = None
     L.ehd_batch_init.argtypes = [vp, C.POINTER(dp)]
+    L.ehd_batch_init_consistent.argtypes = [vp, C.POINTER(dp), C.c_int]
     L.ehd_batch_step.argtypes = [vp, C.c_int]
     L.ehd_batch_diagnostics.argtypes = [vp, C.POINTER(Diag)]
     L.ehd_batch_run_to_steady.argtypes = [vp, C.c_int, C.c_double, C.c_longlong, C.POINTER(Steady)]
@@ -250,6 +253,12 @@ class Box:
             if a.size != 9 * self.S:
                 raise ValueError("fields9 must hold 9*S doubles")
         _check(lib().ehd_init(self._h, _dp(a)))
+
+    def init_consistent(self, fields9: np.ndarray, iterations: int):
+        a = np.ascontiguousarray(fields9, dtype=np.float64)
+        if a.size != 9 * self.S:
+            raise ValueError("fields9 must hold 9*S doubles")
+        _check(lib().ehd_init_consistent(self._h, _dp(a), int(iterations)))
 
     def step(self, n: int = 1):
         _check(lib().ehd_step(self._h, int(n)))
@@ -413,6 +422,11 @@ class Batch:
         keep = [None if f is None else np.ascontiguousarray(f, dtype=np.float64) for f in fields]
         ptrs = (C.POINTER(C.c_double) * len(keep))(*[_dp(a) if a is not None else None for a in keep])
         _check(lib().ehd_batch_init(self._h, ptrs))
+
+    def init_consistent(self, fields, iterations: int):
+        keep = [None if f is None else np.ascontiguousarray(f, dtype=np.float64) for f in fields]
+        ptrs = (C.POINTER(C.c_double) * len(keep))(*[_dp(a) if a is not None else None for a in keep])
+        _check(lib().ehd_batch_init_consistent(self._h, ptrs, int(iterations)))
 
     def step(self, n: int = 1):
         _check(lib().ehd_batch_step(self._h, int(n)))

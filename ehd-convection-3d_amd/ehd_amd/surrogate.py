@@ -12,8 +12,12 @@ Protocol (the acceptance rule of the surrogate):
     the deviations of q and phi from the hydrostatic profile in units of the
     injected charge q0 and the voltage V;
   * evaluation: for held-out (T, C) draws, a cold box (hydrostatic start +
-    noise) and a warm box (the network's fields + the same noise) run to
-    steady in one batch; a warm start is accepted if its final Nu is within
+    noise) and a warm box run to steady in one batch.  The warm box starts
+    from the network's fields by the consistent start (`ehd_init_consistent`:
+    a few thousand steps that collide towards the predicted velocity and
+    charge, building the populations' non-equilibrium parts; an equilibrium
+    start from even the exact steady fields needs ~5e4 steps to satisfy the
+    detector again).  Those steps count for both boxes; a warm start is accepted if its final Nu is within
     0.5 % of the cold start's, otherwise the cold start is the answer.  The
     gain is the ratio of the steps to steady.
 
@@ -229,24 +233,35 @@ def _diag(box):
 
 
 def run_batch_to_steady(params, fields=None, check_every=1000, tol=1e-5, max_steps=200000, min_steps=0,
-                        confirm=2, tag=""):
+                        confirm=2, tag="", consistent=0):
     """Run boxes to steady state in one batch; returns per box (steps,
     converged, Nu) and the final fields.  A box is steady once, over
     `confirm` consecutive windows of check_every steps, both |dNu| < tol and
     |d umax| < tol * max(umax, 1e-6 K V/Lz): near the hydrostatic start Nu
     hardly moves while the instability grows, but umax changes by its growth
     rate, so the start cannot pass for steady.  min_steps (optional) delays
-    the test.  Progress goes to stderr every half minute."""
+    the test.  consistent > 0: the boxes with fields start by
+    `ehd_batch_init_consistent` with that many iterations (every box then
+    has taken that many steps, and they count).  Progress goes to stderr
+    every half minute."""
     n = len(params)
     ufloor = [1e-6 * scales(p)[0] for p in params]
     with Batch(params) as bt:
-        bt.init(fields)
+        if consistent > 0 and fields is not None:
+            bt.init_consistent(fields, consistent)
+            taken = consistent
+        else:
+            bt.init(fields)
+            taken = 0
         d0 = [_diag(b) for b in bt.boxes]
-        prev = [d.nusselt for d in d0]
-        prevu = [d.umax for d in d0]
+        prev = [d.nusselt if d else math.nan for d in d0]
+        prevu = [d.umax if d else math.nan for d in d0]
         res = [Steady(max_steps, False, v) for v in prev]
+        for r, d in zip(res, d0):
+            if d is None:
+                r.failed, r.steps = True, taken
         quiet = [0] * n
-        taken, t0, tp = 0, time.perf_counter(), time.perf_counter()
+        t0, tp = time.perf_counter(), time.perf_counter()
         while taken < max_steps and not all(r.converged or r.failed for r in res):
             k = min(check_every, max_steps - taken)
             bt.step(k)
@@ -293,6 +308,9 @@ def main(argv=None):
     ap.add_argument("--width", type=int, default=16)
     ap.add_argument("--modes", type=int, default=8)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--consistent", type=int, default=2000,
+                    help="consistent-start iterations of the warm boxes (0: equilibrium start); counted as steps "
+                         "of every box of the evaluation batch")
     args = ap.parse_args(argv)
     n = tuple(args.grid)
     rng = np.random.default_rng(args.seed)
@@ -356,7 +374,7 @@ def main(argv=None):
             print(f"epoch {ep}: loss {losses[-1]:.4e} ({time.perf_counter() - t1:.0f} s)", file=sys.stderr, flush=True)
     t_train = time.perf_counter() - t1
 
-    # 3. steps to steady: cold (hydrostatic + noise) vs warm (network fields + the same noise), one batch
+    # 3. steps to steady: cold (hydrostatic + noise) vs warm (the network's fields), one batch
     params, fields = [], []
     for i, (T, C) in enumerate(test_draws):
         seed = 50000 + i
@@ -364,10 +382,11 @@ def main(argv=None):
         y = net.forward(features(n, T, C)[None])[0]
         pw = box_params(n, T, C, seed=seed, init=INIT_FIELDS)
         params += [pc, pw]
-        fields += [None, fields9_from_prediction(pw, y, prof_of(pw), noise_seed=seed, amp=1e-3)]
+        fields += [None, fields9_from_prediction(pw, y, prof_of(pw))]
     t2 = time.perf_counter()
     res, _ = run_batch_to_steady(params, fields, check_every=args.check_every, tol=args.tol,
-                                 max_steps=args.max_steps, min_steps=0, tag="evaluation:")
+                                 max_steps=args.max_steps, min_steps=0, tag="evaluation:",
+                                 consistent=args.consistent)
     t_eval = time.perf_counter() - t2
     rows = []
     for i, (T, C) in enumerate(test_draws):
@@ -384,6 +403,8 @@ def main(argv=None):
         "generation_steps_median": int(np.median(gen_steps)),
         "final_train_loss": losses[-1], "first_epoch_loss": losses[0],
         "T_range": list(args.T_range), "C_range": list(args.C_range), "epochs": args.epochs,
+        "check_every": args.check_every, "tol": args.tol, "max_steps": args.max_steps,
+        "width": args.width, "modes": args.modes, "consistent_iterations": args.consistent,
         "train_draws": len(train_draws),
         "accepted": len(acc), "acceptance_rate": round(len(acc) / len(rows), 3),
         "steps_ratio_median_accepted": round(float(np.median(speed)), 3) if speed else None,

@@ -114,6 +114,17 @@ void ehd_destroy(ehd_box* b);   /* main.cu:283-306 frees */
  * required for EHD_INIT_FIELDS and ignored otherwise. */
 int ehd_init(ehd_box* b, const double* fields9);
 
+/* Consistent start from given fields (no reference counterpart; after Mei et
+ * al., J. Comput. Phys. 219 (2006) 2011, for warm starts): ehd_init with
+ * EHD_INIT_FIELDS, then `iterations` coupled steps in which every interior
+ * node collides towards the equilibria of the given velocity and charge
+ * (its density free, the Poisson solve on the given charge), so that the
+ * populations gain the non-equilibrium parts of that state; the equilibrium
+ * start alone lacks them and its relaxation excites the slow modes.  The
+ * iterations count as steps (step and t advance).  FP32S boxes with
+ * init_mode EHD_INIT_FIELDS. */
+int ehd_init_consistent(ehd_box* b, const double* fields9, int iterations);
+
 /* nsteps coupled steps, each = stream_collide_save + extension + FFT +
  * derivative + IFFT + extract + efield (main.cu:181-208). */
 int ehd_step(ehd_box* b, int nsteps);
@@ -236,6 +247,10 @@ void ehd_batch_destroy(ehd_batch* bt);
 /* ehd_init of every box; fields9[i] as ehd_init's argument (fields9 itself may be NULL). */
 int ehd_batch_init(ehd_batch* bt, const double* const* fields9);
 int ehd_batch_step(ehd_batch* bt, int nsteps);
+/* ehd_init_consistent for the boxes with fields9[i] != NULL (the others start
+ * as ehd_init does and step normally meanwhile); every box advances
+ * `iterations` steps.  FP32S batches. */
+int ehd_batch_init_consistent(ehd_batch* bt, const double* const* fields9, int iterations);
 /* ehd_diagnostics of every box (out: nboxes entries) */
 int ehd_batch_diagnostics(ehd_batch* bt, ehd_diag* out);
 /* steps the batch in chunks of check_every until every box has |dNu| < tol

@@ -59,8 +59,8 @@ def test_warm_start_gain():
     for i, (T, y) in enumerate(zip(TEST_T, pred)):
         pw = S.box_params(GRID, T, C, seed=900 + i, init=S.INIT_FIELDS)
         warm_params.append(pw)
-        warm_fields.append(S.fields9_from_prediction(pw, y, S.hydrostatic_profile(pw), noise_seed=900 + i, amp=1e-3))
-    res_warm, _ = S.run_batch_to_steady(warm_params, warm_fields, tag="warm", **RUN)
+        warm_fields.append(S.fields9_from_prediction(pw, y, S.hydrostatic_profile(pw)))
+    res_warm, _ = S.run_batch_to_steady(warm_params, warm_fields, tag="warm", consistent=2000, **RUN)
     cold = sum(r.steps for r in res_cold)
     warm = sum(r.steps for r in res_warm)
     for T, c, w in zip(TEST_T, res_cold, res_warm):
