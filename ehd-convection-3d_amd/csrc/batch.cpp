@@ -53,12 +53,11 @@ struct ehd_batch {
 
 namespace {
 
-// titems: TARGET item tables per parity (consistent start), or null
-int batch_lbm(ehd_batch* bt, int cur, bool first, bool ebuf, bool macro, void* const* titems = nullptr)
+int batch_lbm(ehd_batch* bt, int cur, bool first, bool ebuf, bool macro)
 {
     if (bt->prec == Prec::F32S) {
-        HIPCHK(launch_lbm_f32s_batch(titems ? titems[cur] : bt->ditems[cur], (int)bt->boxes.size(), bt->total_tiles,
-                                     bt->lbm_grid, bt->lbm_chunk, first, ebuf, macro, bt->st, titems != nullptr));
+        HIPCHK(launch_lbm_f32s_batch(bt->ditems[cur], (int)bt->boxes.size(), bt->total_tiles, bt->lbm_grid,
+                                     bt->lbm_chunk, first, ebuf, macro, bt->st));
     } else {
         for (ehd_box* b : bt->boxes)
             HIPCHK(launch_lbm_f64(b->lp, pop_bufs_at(b, cur), macro_bufs(b), first, ebuf, macro, bt->st));
@@ -346,20 +345,16 @@ int ehd_batch_init_consistent(ehd_batch* bt, const double* const* fields9, int i
     if (rc || iterations == 0) return rc;
     // targets of the boxes that have fields; TARGET item tables of the whole batch
     std::vector<double*> tg(n, nullptr);
-    void* titems[2] = {nullptr, nullptr};
     struct Release {
         std::vector<double*>& tg;
-        void** it;
         hipStream_t st;
         ~Release()
         {
             (void)hipStreamSynchronize(st);
             for (double* p : tg)
                 if (p) (void)hipFree(p);
-            for (int c = 0; c < 2; ++c)
-                if (it[c]) (void)hipFree(it[c]);
         }
-    } release{tg, titems, bt->st};
+    } release{tg, bt->st};
     const int src[4] = {3, 4, 5, 1};   // fields9 -> ux, uy, uz, q
     for (size_t i = 0; i < n; ++i) {
         if (!fields9[i]) continue;
@@ -372,27 +367,60 @@ int ehd_batch_init_consistent(ehd_batch* bt, const double* const* fields9, int i
             HIPCHK(hipMemcpyAsync(tg[i] + k * S, fields9[i] + src[k] * S, S * sizeof(double), hipMemcpyHostToDevice,
                                   bt->st));
     }
+    // two launches per iteration: the targeted boxes through the TARGET
+    // kernel, the others through the ordinary kernel (so that they step
+    // exactly as they do in ehd_batch_step); each set has its own item
+    // tables and tile numbering
     const size_t isz = lbm_f32s_batch_item_size();
-    for (int cur = 0; cur < 2; ++cur) {
-        std::vector<unsigned char> h(isz * n);
-        unsigned tile0 = 0;
-        for (size_t i = 0; i < n; ++i) {
-            ehd_box* b = bt->boxes[i];
-            lbm_f32s_batch_item(h.data() + i * isz, b->lp, pop_bufs_at(b, cur), macro_bufs(b), tile0, tg[i]);
-            tile0 += (unsigned)((b->S + 255) / 256);
+    struct Set {
+        void* items[2] = {nullptr, nullptr};
+        int n = 0;
+        unsigned tiles = 0;
+    } sets[2];   // 0: ordinary, 1: targeted
+    struct ReleaseSets {
+        Set* s;
+        hipStream_t st;
+        ~ReleaseSets()
+        {
+            (void)hipStreamSynchronize(st);
+            for (int k = 0; k < 2; ++k)
+                for (int c = 0; c < 2; ++c)
+                    if (s[k].items[c]) (void)hipFree(s[k].items[c]);
         }
-        if (hipMalloc(&titems[cur], h.size()) != hipSuccess) {
-            titems[cur] = nullptr;
-            return set_error(EHD_ENOMEM, "item table allocation failed");
+    } release_sets{sets, bt->st};
+    for (int k = 0; k < 2; ++k) {
+        for (int cur = 0; cur < 2; ++cur) {
+            std::vector<unsigned char> h;
+            unsigned tile0 = 0;
+            int m = 0;
+            for (size_t i = 0; i < n; ++i) {
+                if ((tg[i] != nullptr) != (k == 1)) continue;
+                ehd_box* b = bt->boxes[i];
+                h.resize(isz * (m + 1));
+                lbm_f32s_batch_item(h.data() + m * isz, b->lp, pop_bufs_at(b, cur), macro_bufs(b), tile0, tg[i]);
+                tile0 += (unsigned)((b->S + 255) / 256);
+                ++m;
+            }
+            sets[k].n = m;
+            sets[k].tiles = tile0;
+            if (m == 0) continue;
+            if (hipMalloc(&sets[k].items[cur], h.size()) != hipSuccess) {
+                sets[k].items[cur] = nullptr;
+                return set_error(EHD_ENOMEM, "item table allocation failed");
+            }
+            HIPCHK(hipMemcpyAsync(sets[k].items[cur], h.data(), h.size(), hipMemcpyHostToDevice, bt->st));
+            HIPCHK(hipStreamSynchronize(bt->st));
         }
-        HIPCHK(hipMemcpyAsync(titems[cur], h.data(), h.size(), hipMemcpyHostToDevice, bt->st));
-        HIPCHK(hipStreamSynchronize(bt->st));
     }
     for (ehd_box* b : bt->boxes) HIPCHK(hipStreamSynchronize(b->st));
     int cur = bt->boxes[0]->cur;
     bool direct = bt->boxes[0]->direct, ebuf = bt->boxes[0]->ebuf;
     for (int i = 0; i < iterations; ++i) {
-        if ((rc = batch_lbm(bt, cur, direct, ebuf, i == iterations - 1, titems))) return rc;
+        const bool macro = (i == iterations - 1);
+        for (int k = 0; k < 2; ++k)
+            if (sets[k].n > 0)
+                HIPCHK(launch_lbm_f32s_batch(sets[k].items[cur], sets[k].n, sets[k].tiles, 0, bt->lbm_chunk, direct,
+                                             ebuf, macro, bt->st, k == 1));
         if ((rc = batch_poisson(bt))) return rc;
         cur = 1 - cur;
         direct = false;

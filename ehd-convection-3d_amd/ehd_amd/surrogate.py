@@ -9,8 +9,9 @@ Protocol (the acceptance rule of the surrogate):
     box picks its own pattern position (and, for rolls, direction), which the
     network cannot know;
   * targets: the steady u, v, w in units of the drift velocity K V / Lz and
-    the deviations of q and phi from the hydrostatic profile in units of the
-    injected charge q0 and the voltage V;
+    the deviations of q and phi from the hydrostatic profile in units of
+    0.2 q0 and 0.05 V (the injected charge and the voltage, scaled so that
+    every channel has an rms of about 0.5 and weighs alike in the loss);
   * evaluation: for held-out (T, C) draws, a cold box (hydrostatic start +
     noise) and a warm box run to steady in one batch.  The warm box starts
     from the network's fields by the consistent start (`ehd_init_consistent`:
@@ -50,6 +51,10 @@ from .fno import FNO
 
 D_IN, D_OUT = 6, 5
 CFL = 0.04          # dt/dx of the surrogate's boxes
+# units of the q and phi targets, in q0 and V: the deviations from the
+# hydrostatic profile are ~0.1 q0 and ~0.025 V where the velocities are
+# ~0.5-1.4 drift velocities
+Q_UNIT, PHI_UNIT = 0.2, 0.05
 
 
 def box_params(n, T, C, M=10.0, seed=1, init=INIT_HYDROSTATIC, amp=1e-3, cfl=CFL, pattern=PERTURB_NOISE):
@@ -91,8 +96,8 @@ def targets_from_fields(p, f, prof):
     qz, phz = prof
     t = np.empty((D_OUT,) + f["ux"].shape, np.float32)
     t[0], t[1], t[2] = f["ux"] / U, f["uy"] / U, f["uz"] / U
-    t[3] = (f["q"] - qz[:, None, None]) / Q
-    t[4] = (f["phi"] - phz[:, None, None]) / V
+    t[3] = (f["q"] - qz[:, None, None]) / (Q_UNIT * Q)
+    t[4] = (f["phi"] - phz[:, None, None]) / (PHI_UNIT * V)
     # wall rows of the macroscopic outputs are raw collide values: use the extrapolation every consumer uses
     for k in range(4):
         t[k, 0] = 2 * t[k, 1] - t[k, 2]
@@ -192,13 +197,13 @@ def fields9_from_prediction(p, y, prof, noise_seed=None, amp=0.0):
     nz, ny, nx = p.nz, p.ny, p.nx
     f = np.zeros((9, nz, ny, nx))
     f[0] = p.rho0
-    f[1] = qz[:, None, None] + Q * y[3]
+    f[1] = qz[:, None, None] + Q_UNIT * Q * y[3]
     # The injection wall takes the injected charge, not a prediction: its
     # charge populations are rebuilt from their own previous values every step
     # (LBM.cu:1281-1353), so q(z=0) follows q' = 2 q_inj - q and a start away
     # from q_inj oscillates with period 2 for ever and shifts the steady state.
     f[1][0] = qz[0]
-    f[2] = phz[:, None, None] + V * y[4]
+    f[2] = phz[:, None, None] + PHI_UNIT * V * y[4]
     f[2][0] = p.voltage
     f[2][-1] = 0.0
     f[3], f[4], f[5] = U * y[0], U * y[1], U * y[2]
@@ -303,12 +308,12 @@ def main(argv=None):
                     help="training and test range of T (steady convection on this grid)")
     ap.add_argument("--C-range", type=float, nargs=2, default=(5.0, 20.0))
     ap.add_argument("--min-steps", type=int, default=0)
-    ap.add_argument("--epochs", type=int, default=300)
-    ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--width", type=int, default=16)
-    ap.add_argument("--modes", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=400)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--width", type=int, default=32)
+    ap.add_argument("--modes", type=int, nargs=3, default=(8, 8, 16), help="Fourier modes along x, y, z")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--consistent", type=int, default=2000,
+    ap.add_argument("--consistent", type=int, default=5000,
                     help="consistent-start iterations of the warm boxes (0: equilibrium start); counted as steps "
                          "of every box of the evaluation batch")
     args = ap.parse_args(argv)
@@ -352,7 +357,7 @@ def main(argv=None):
     t_gen = time.perf_counter() - t0
 
     # 2. training on the GPU (the library's forward/backward kernels, Adam)
-    net = FNO(n[0], n[1], n[2], width=args.width, modes=(args.modes,) * 3, max_batch=8)
+    net = FNO(n[0], n[1], n[2], width=args.width, modes=tuple(args.modes), max_batch=8)
     net.set_params(net.init_params(seed=args.seed))
     ntr = len(X)
     bs = 8
@@ -375,16 +380,17 @@ def main(argv=None):
     t_train = time.perf_counter() - t1
 
     # 3. steps to steady: cold (hydrostatic + noise) vs warm (the network's fields), one batch
-    params, fields = [], []
+    params, fields, preds = [], [], []
     for i, (T, C) in enumerate(test_draws):
         seed = 50000 + i
         pc = box_params(n, T, C, seed=seed)
         y = net.forward(features(n, T, C)[None])[0]
+        preds.append(y)
         pw = box_params(n, T, C, seed=seed, init=INIT_FIELDS)
         params += [pc, pw]
         fields += [None, fields9_from_prediction(pw, y, prof_of(pw))]
     t2 = time.perf_counter()
-    res, _ = run_batch_to_steady(params, fields, check_every=args.check_every, tol=args.tol,
+    res, out = run_batch_to_steady(params, fields, check_every=args.check_every, tol=args.tol,
                                  max_steps=args.max_steps, min_steps=0, tag="evaluation:",
                                  consistent=args.consistent)
     t_eval = time.perf_counter() - t2
@@ -392,9 +398,14 @@ def main(argv=None):
     for i, (T, C) in enumerate(test_draws):
         c, w = res[2 * i], res[2 * i + 1]
         ok = bool(c.converged and w.converged and abs(w.nusselt / c.nusselt - 1.0) < 5e-3)
+        err = None
+        if c.converged:   # the network's distance to the cold run's steady state (canonical position)
+            pc = params[2 * i]
+            yt = canonicalize(targets_from_fields(pc, out[2 * i], prof_of(pc)))
+            err = round(float(np.linalg.norm(preds[i] - yt) / np.linalg.norm(yt)), 5)
         rows.append({"T": round(T, 2), "C": round(C, 3), "cold_steps": c.steps, "warm_steps": w.steps,
                      "cold_Nu": c.nusselt, "warm_Nu": w.nusselt, "cold_failed": c.failed,
-                     "warm_failed": w.failed, "accepted": ok})
+                     "warm_failed": w.failed, "accepted": ok, "prediction_error": err})
     acc = [r for r in rows if r["accepted"]]
     speed = [r["cold_steps"] / max(r["warm_steps"], 1) for r in acc]
     summary = {
@@ -404,9 +415,12 @@ def main(argv=None):
         "final_train_loss": losses[-1], "first_epoch_loss": losses[0],
         "T_range": list(args.T_range), "C_range": list(args.C_range), "epochs": args.epochs,
         "check_every": args.check_every, "tol": args.tol, "max_steps": args.max_steps,
-        "width": args.width, "modes": args.modes, "consistent_iterations": args.consistent,
+        "width": args.width, "modes": list(args.modes), "consistent_iterations": args.consistent,
         "train_draws": len(train_draws),
         "accepted": len(acc), "acceptance_rate": round(len(acc) / len(rows), 3),
+        "prediction_error_median": (float(np.median([r["prediction_error"] for r in rows
+                                                     if r["prediction_error"] is not None]))
+                                    if any(r["prediction_error"] is not None for r in rows) else None),
         "steps_ratio_median_accepted": round(float(np.median(speed)), 3) if speed else None,
         "steps_ratio_mean_accepted": round(float(np.mean(speed)), 3) if speed else None,
         "total_cold_steps": int(sum(r["cold_steps"] for r in rows)),

@@ -38,29 +38,29 @@ def test_warm_start_gain():
     Xtr, Ytr, Xte, Yte = X[:ntr], Y[:ntr], X[ntr:], Y[ntr:]
     res_cold = res[ntr:]
 
-    net = FNO(*GRID, width=16, modes=(8, 8, 8), layers=4, max_batch=ntr)
+    net = FNO(*GRID, width=16, modes=(8, 8, 16), layers=4, max_batch=ntr)
     net.set_params(net.init_params(seed=3))
     losses = []
-    nit = 400
+    nit = 2000
     for it in range(nit):
         lr = 2e-3 * 0.5 * (1 + math.cos(math.pi * it / nit))
         losses.append(net.train_step(Xtr, Ytr, lr))
     print(f"training loss {np.mean(losses[:10]):.3e} -> {np.mean(losses[-10:]):.3e}")
-    assert np.mean(losses[-10:]) < 0.2 * np.mean(losses[:10])
+    assert np.mean(losses[-10:]) < 0.01 * np.mean(losses[:10])
 
     pred = net.forward(Xte)
     # distance to the steady state (canonical phase): network vs the hydrostatic
     # start (zero velocity, hydrostatic q and phi, i.e. all targets zero)
     err_net = float(np.linalg.norm(pred - Yte) / np.linalg.norm(Yte))
     print(f"held-out rel-L2 to the steady state: network {err_net:.3f}, hydrostatic start 1.000")
-    assert err_net < 0.5
+    assert err_net < 0.05
 
     warm_params, warm_fields = [], []
     for i, (T, y) in enumerate(zip(TEST_T, pred)):
         pw = S.box_params(GRID, T, C, seed=900 + i, init=S.INIT_FIELDS)
         warm_params.append(pw)
         warm_fields.append(S.fields9_from_prediction(pw, y, S.hydrostatic_profile(pw)))
-    res_warm, _ = S.run_batch_to_steady(warm_params, warm_fields, tag="warm", consistent=2000, **RUN)
+    res_warm, _ = S.run_batch_to_steady(warm_params, warm_fields, tag="warm", consistent=5000, **RUN)
     cold = sum(r.steps for r in res_cold)
     warm = sum(r.steps for r in res_warm)
     for T, c, w in zip(TEST_T, res_cold, res_warm):
