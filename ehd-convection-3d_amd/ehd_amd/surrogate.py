@@ -313,6 +313,8 @@ def main(argv=None):
     ap.add_argument("--width", type=int, default=32)
     ap.add_argument("--modes", type=int, nargs=3, default=(8, 8, 16), help="Fourier modes along x, y, z")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--save-net", default="", help="save the trained parameters (.npy)")
+    ap.add_argument("--load-net", default="", help="skip generation and training: evaluate these parameters")
     ap.add_argument("--consistent", type=int, default=5000,
                     help="consistent-start iterations of the warm boxes (0: equilibrium start); counted as steps "
                          "of every box of the evaluation batch")
@@ -336,48 +338,61 @@ def main(argv=None):
             profs[key] = hydrostatic_profile(p)
         return profs[key]
 
-    # 1. training states from the batched solver
-    X, Y = [], []
-    gen_steps = []
-    for s in range(0, len(train_draws), args.batch_boxes):
-        chunk = train_draws[s:s + args.batch_boxes]
-        params = [box_params(n, T, C, seed=1000 + s + i) for i, (T, C) in enumerate(chunk)]
-        res, fields = run_batch_to_steady(params, check_every=args.check_every, tol=args.tol,
-                                          max_steps=args.max_steps, min_steps=args.min_steps,
-                                          tag=f"train batch {s // args.batch_boxes}:")
-        for p, (T, C), f, r in zip(params, chunk, fields, res):
-            if not r.converged or not (0.5 < r.nusselt < 5.0):
-                continue     # not steady (or the lattice went unstable): not a training state
-            X.append(features(n, T, C))
-            Y.append(canonicalize(targets_from_fields(p, f, prof_of(p))))
-            gen_steps.append(r.steps)
-        print(f"generated {len(X)} states ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
-    X = np.stack(X)
-    Y = np.stack(Y)
-    t_gen = time.perf_counter() - t0
-
-    # 2. training on the GPU (the library's forward/backward kernels, Adam)
     net = FNO(n[0], n[1], n[2], width=args.width, modes=tuple(args.modes), max_batch=8)
-    net.set_params(net.init_params(seed=args.seed))
-    ntr = len(X)
-    bs = 8
-    steps_per_epoch = max(1, ntr // bs)
-    total = args.epochs * steps_per_epoch
-    losses = []
-    t1 = time.perf_counter()
-    k = 0
-    for ep in range(args.epochs):
-        order = rng.permutation(ntr)
-        el = []
-        for j in range(steps_per_epoch):
-            idx = order[j * bs:(j + 1) * bs]
-            lr = args.lr * 0.5 * (1 + math.cos(math.pi * k / total))
-            el.append(net.train_step(X[idx], Y[idx], lr))
-            k += 1
-        losses.append(float(np.mean(el)))
-        if ep % 5 == 0 or ep == args.epochs - 1:
-            print(f"epoch {ep}: loss {losses[-1]:.4e} ({time.perf_counter() - t1:.0f} s)", file=sys.stderr, flush=True)
-    t_train = time.perf_counter() - t1
+    X, gen_steps, losses, ntr, t_gen, t_train = None, [], [], 0, 0.0, 0.0
+    if args.load_net:
+        # a network trained by an earlier run with the same grid, width and modes
+        net.set_params(np.load(args.load_net))
+    else:
+        # 1. training states from the batched solver
+        X, Y = [], []
+        gen_steps = []
+        for s in range(0, len(train_draws), args.batch_boxes):
+            chunk = train_draws[s:s + args.batch_boxes]
+            params = [box_params(n, T, C, seed=1000 + s + i) for i, (T, C) in enumerate(chunk)]
+            res, fields = run_batch_to_steady(params, check_every=args.check_every, tol=args.tol,
+                                              max_steps=args.max_steps, min_steps=args.min_steps,
+                                              tag=f"train batch {s // args.batch_boxes}:")
+            for p, (T, C), f, r in zip(params, chunk, fields, res):
+                if not r.converged or not (0.5 < r.nusselt < 5.0):
+                    continue     # not steady (or the lattice went unstable): not a training state
+                X.append(features(n, T, C))
+                Y.append(canonicalize(targets_from_fields(p, f, prof_of(p))))
+                gen_steps.append(r.steps)
+            print(f"generated {len(X)} states ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+        X = np.stack(X)
+        Y = np.stack(Y)
+        t_gen = time.perf_counter() - t0
+
+        # 2. training on the GPU (the library's forward/backward kernels, Adam)
+        net.set_params(net.init_params(seed=args.seed))
+        ntr = len(X)
+        bs = 8
+        steps_per_epoch = max(1, ntr // bs)
+        total = args.epochs * steps_per_epoch
+        losses = []
+        t1 = time.perf_counter()
+        k = 0
+        for ep in range(args.epochs):
+            order = rng.permutation(ntr)
+            el = []
+            for j in range(steps_per_epoch):
+                idx = order[j * bs:(j + 1) * bs]
+                lr = args.lr * 0.5 * (1 + math.cos(math.pi * k / total))
+                el.append(net.train_step(X[idx], Y[idx], lr))
+                k += 1
+            losses.append(float(np.mean(el)))
+            if ep % 5 == 0 or ep == args.epochs - 1:
+                print(f"epoch {ep}: loss {losses[-1]:.4e} ({time.perf_counter() - t1:.0f} s)", file=sys.stderr, flush=True)
+        t_train = time.perf_counter() - t1
+        if args.save_net:
+            np.save(args.save_net, net.get_params())
+
+    if not test_draws:
+        print(json.dumps({"grid": list(n), "train_states": int(ntr), "generation_s": round(t_gen, 1),
+                          "training_s": round(t_train, 1), "final_train_loss": losses[-1] if losses else None,
+                          "saved": args.save_net or None}), flush=True)
+        return 0
 
     # 3. steps to steady: cold (hydrostatic + noise) vs warm (the network's fields), one batch
     params, fields, preds = [], [], []
@@ -411,8 +426,9 @@ def main(argv=None):
     summary = {
         "grid": list(n), "train_states": int(ntr), "test_draws": len(test_draws),
         "generation_s": round(t_gen, 1), "training_s": round(t_train, 1), "evaluation_s": round(t_eval, 1),
-        "generation_steps_median": int(np.median(gen_steps)),
-        "final_train_loss": losses[-1], "first_epoch_loss": losses[0],
+        "generation_steps_median": int(np.median(gen_steps)) if gen_steps else None,
+        "network": args.load_net or "trained in this run",
+        "final_train_loss": losses[-1] if losses else None, "first_epoch_loss": losses[0] if losses else None,
         "T_range": list(args.T_range), "C_range": list(args.C_range), "epochs": args.epochs,
         "check_every": args.check_every, "tol": args.tol, "max_steps": args.max_steps,
         "width": args.width, "modes": list(args.modes), "consistent_iterations": args.consistent,
