@@ -13,7 +13,8 @@ Protocol (the acceptance rule of the surrogate):
     0.2 q0 and 0.05 V (the injected charge and the voltage, scaled so that
     every channel has an rms of about 0.5 and weighs alike in the loss);
   * evaluation: for held-out (T, C) draws, a cold box (hydrostatic start +
-    noise) and a warm box run to steady in one batch.  The warm box starts
+    noise) and a warm box run to steady (the cold boxes in one batch, the
+    warm ones in another).  The warm box starts
     from the network's fields by the consistent start (`ehd_init_consistent`:
     a few thousand steps that collide towards the predicted velocity and
     charge, building the populations' non-equilibrium parts; an equilibrium
@@ -304,6 +305,8 @@ def main(argv=None):
     ap.add_argument("--check-every", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--max-steps", type=int, default=300000)
+    ap.add_argument("--gen-max-steps", type=int, default=None,
+                    help="step limit of the training-state generation (default: --max-steps)")
     ap.add_argument("--T-range", type=float, nargs=2, default=(200.0, 300.0),
                     help="training and test range of T (steady convection on this grid)")
     ap.add_argument("--C-range", type=float, nargs=2, default=(5.0, 20.0))
@@ -314,6 +317,8 @@ def main(argv=None):
     ap.add_argument("--modes", type=int, nargs=3, default=(8, 8, 16), help="Fourier modes along x, y, z")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--save-net", default="", help="save the trained parameters (.npy)")
+    ap.add_argument("--test-seed", type=int, default=None, help="draw the held-out (T, C) from their own generator")
+    ap.add_argument("--test-offset", type=int, default=0, help="skip this many draws of that generator")
     ap.add_argument("--load-net", default="", help="skip generation and training: evaluate these parameters")
     ap.add_argument("--consistent", type=int, default=5000,
                     help="consistent-start iterations of the warm boxes (0: equilibrium start); counted as steps "
@@ -329,7 +334,18 @@ def main(argv=None):
         return [(float(rng.uniform(t0_, t1_)), float(math.exp(rng.uniform(math.log(c0_), math.log(c1_)))))
                 for _ in range(k)]
 
-    train_draws, test_draws = draws(args.train), draws(args.test)
+    train_draws = draws(args.train)
+    if args.test_seed is None:
+        test_draws = draws(args.test)
+    else:
+        # held-out draws from their own generator: several runs can evaluate
+        # disjoint slices of one draw sequence against the same (deterministic) training
+        rng_t = np.random.default_rng(args.test_seed)
+        t0_, t1_ = args.T_range
+        c0_, c1_ = args.C_range
+        seq = [(float(rng_t.uniform(t0_, t1_)), float(math.exp(rng_t.uniform(math.log(c0_), math.log(c1_)))))
+               for _ in range(args.test_offset + args.test)]
+        test_draws = seq[args.test_offset:]
     profs = {}
 
     def prof_of(p):
@@ -351,7 +367,8 @@ def main(argv=None):
             chunk = train_draws[s:s + args.batch_boxes]
             params = [box_params(n, T, C, seed=1000 + s + i) for i, (T, C) in enumerate(chunk)]
             res, fields = run_batch_to_steady(params, check_every=args.check_every, tol=args.tol,
-                                              max_steps=args.max_steps, min_steps=args.min_steps,
+                                              max_steps=args.gen_max_steps or args.max_steps,
+                                              min_steps=args.min_steps,
                                               tag=f"train batch {s // args.batch_boxes}:")
             for p, (T, C), f, r in zip(params, chunk, fields, res):
                 if not r.converged or not (0.5 < r.nusselt < 5.0):
@@ -394,21 +411,29 @@ def main(argv=None):
                           "saved": args.save_net or None}), flush=True)
         return 0
 
-    # 3. steps to steady: cold (hydrostatic + noise) vs warm (the network's fields), one batch
-    params, fields, preds = [], [], []
+    # 3. steps to steady: cold (hydrostatic + noise) and warm (the network's fields), each
+    # set in a batch of its own (a batch steps until its slowest box is steady)
+    cparams, wparams, wfields, preds = [], [], [], []
     for i, (T, C) in enumerate(test_draws):
-        seed = 50000 + i
-        pc = box_params(n, T, C, seed=seed)
+        seed = 50000 + args.test_offset + i
         y = net.forward(features(n, T, C)[None])[0]
         preds.append(y)
+        cparams.append(box_params(n, T, C, seed=seed))
         pw = box_params(n, T, C, seed=seed, init=INIT_FIELDS)
-        params += [pc, pw]
-        fields += [None, fields9_from_prediction(pw, y, prof_of(pw))]
+        wparams.append(pw)
+        wfields.append(fields9_from_prediction(pw, y, prof_of(pw)))
     t2 = time.perf_counter()
-    res, out = run_batch_to_steady(params, fields, check_every=args.check_every, tol=args.tol,
-                                 max_steps=args.max_steps, min_steps=0, tag="evaluation:",
-                                 consistent=args.consistent)
+    wres, _ = run_batch_to_steady(wparams, wfields, check_every=args.check_every, tol=args.tol,
+                                  max_steps=args.max_steps, min_steps=0, tag="evaluation (warm):",
+                                  consistent=args.consistent)
+    cres, cout = run_batch_to_steady(cparams, check_every=args.check_every, tol=args.tol,
+                                     max_steps=args.max_steps, min_steps=0, tag="evaluation (cold):")
     t_eval = time.perf_counter() - t2
+    res, out, params = [], [], []
+    for i in range(len(test_draws)):
+        res += [cres[i], wres[i]]
+        out += [cout[i], None]
+        params += [cparams[i], wparams[i]]
     rows = []
     for i, (T, C) in enumerate(test_draws):
         c, w = res[2 * i], res[2 * i + 1]
@@ -430,7 +455,9 @@ def main(argv=None):
         "network": args.load_net or "trained in this run",
         "final_train_loss": losses[-1] if losses else None, "first_epoch_loss": losses[0] if losses else None,
         "T_range": list(args.T_range), "C_range": list(args.C_range), "epochs": args.epochs,
+        "test_seed": args.test_seed, "test_offset": args.test_offset,
         "check_every": args.check_every, "tol": args.tol, "max_steps": args.max_steps,
+        "gen_max_steps": args.gen_max_steps or args.max_steps,
         "width": args.width, "modes": list(args.modes), "consistent_iterations": args.consistent,
         "train_draws": len(train_draws),
         "accepted": len(acc), "acceptance_rate": round(len(acc) / len(rows), 3),
