@@ -1,8 +1,9 @@
 // lbm_f32s.hip -- fused pull-scheme LBM step with FP32 storage of shifted
 // populations (the performance path).
 //
-// Storage: g_d = f_d - w_d*rs(z) and k_d = h_d - w_d*qs(z), float, SoA
-// [d-1][z][y][x], double buffered, where rs(z), qs(z) are the hydrostatic
+// Storage: g_d = f_d - w_d*rs(z) and k_d = h_d - w_d*qs(z), float, in row
+// blocks [z][y][d-1][x] (the 26 moving populations of one x-row together;
+// rest populations [z][y][x] apart), double buffered, where rs(z), qs(z) are the hydrostatic
 // density and charge profiles of the box (fixed at creation).  The shifts take
 // the large, smooth part of every population out of the float rounding: what
 // is stored is the deviation from the hydrostatic state, so the rounding of
@@ -89,8 +90,11 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
     const unsigned ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
     const unsigned zm = (z == 0) ? nz - 1 : z - 1, zp = (z == nz - 1) ? 0 : z + 1;
     const unsigned XS[3] = {xm, x, xp};
-    const unsigned YS[3] = {ym * nx, y * nx, yp * nx};
-    const unsigned ZS[3] = {zm * nxy, z * nxy, zp * nxy};
+    // populations: row blocks [z][y][d-1][x] (26 x nx floats per row); the
+    // source row of a pull along c is (y - c_y, z - c_z)
+    const unsigned RW = 26u * nx;
+    const unsigned YR[3] = {ym, y, yp}, ZR[3] = {zm * ny, z * ny, zp * ny};
+    const unsigned own = t * RW + x;
 
     cgfloat* __restrict__ sh = gc(P.shf);
     const float rs = sh[z], dRu = sh[nz + z], dRd = sh[2 * nz + z];
@@ -107,23 +111,24 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
         k[0] = h0A[s];
 #pragma unroll
         for (int d = 1; d < 27; ++d) {
-            g[d] = fA[(d - 1) * S + s];
-            k[d] = hA[(d - 1) * S + s];
+            g[d] = fA[own + (d - 1) * nx];
+            k[d] = hA[own + (d - 1) * nx];
         }
     } else {
 #pragma unroll
         for (int d = 1; d < 27; ++d)
-            g[d] = pulled(ldnt(fA + (d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]), d, c.w[WCLS[d]], dRu, dRd);
+            g[d] = pulled(ldnt(fA + (ZR[1 - CZ[d]] + YR[1 - CY[d]]) * RW + (d - 1) * nx + XS[1 - CX[d]]), d,
+                          c.w[WCLS[d]], dRu, dRd);
         if (z == 0) {
             k[0] = -h0A[s] + c.injk[0];
 #pragma unroll
-            for (int d = 1; d < 27; ++d) k[d] = -hA[(opp(d) - 1) * S + s] + c.injk[WCLS[d]];
+            for (int d = 1; d < 27; ++d) k[d] = -hA[own + (opp(d) - 1) * nx] + c.injk[WCLS[d]];
         } else {
             k[0] = h0A[s];
 #pragma unroll
             for (int d = 1; d < 27; ++d)
-                k[d] = pulled(ldnt(hA + (d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]), d, c.w[WCLS[d]],
-                              dQu, dQd);
+                k[d] = pulled(ldnt(hA + (ZR[1 - CZ[d]] + YR[1 - CY[d]]) * RW + (d - 1) * nx + XS[1 - CX[d]]), d,
+                              c.w[WCLS[d]], dQu, dQd);
         }
     }
 
@@ -185,19 +190,20 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
         const unsigned s1 = s + nxy;
         float dqm = h0A[s1];
         if (FIRST) {
+            const unsigned own1 = own + ny * RW;
 #pragma unroll
             for (int d = 1; d < 27; ++d) {
-                gm[d] = fA[(d - 1) * S + s1];
-                dqm += hA[(d - 1) * S + s1];
+                gm[d] = fA[own1 + (d - 1) * nx];
+                dqm += hA[own1 + (d - 1) * nx];
             }
         } else {
-            const unsigned Z1[3] = {0u, nxy, 2u * nxy};
             const float dRu1 = sh[nz + 1], dRd1 = sh[2 * nz + 1], dQu1 = sh[4 * nz + 1], dQd1 = sh[5 * nz + 1];
 #pragma unroll
             for (int d = 1; d < 27; ++d) {
-                const unsigned src = XS[1 - CX[d]] + YS[1 - CY[d]] + Z1[1 - CZ[d]];
-                gm[d] = pulled(fA[(d - 1) * S + src], d, c.w[WCLS[d]], dRu1, dRd1);
-                dqm += pulled(hA[(d - 1) * S + src], d, c.w[WCLS[d]], dQu1, dQd1);
+                // node (x, y, 1) pulls from row (y - c_y, 1 - c_z)
+                const unsigned src = ((unsigned)(1 - CZ[d]) * ny + YR[1 - CY[d]]) * RW + (d - 1) * nx + XS[1 - CX[d]];
+                gm[d] = pulled(fA[src], d, c.w[WCLS[d]], dRu1, dRd1);
+                dqm += pulled(hA[src], d, c.w[WCLS[d]], dQu1, dQd1);
             }
         }
         float Exm, Eym, Ezm;
@@ -302,14 +308,14 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
             od = gd;
             oe = ge;
         }
-        stnt(od, fB + (d - 1) * S + s);
-        stnt(oe, fB + (e - 1) * S + s);
+        stnt(od, fB + own + (d - 1) * nx);
+        stnt(oe, fB + own + (e - 1) * nx);
         if (z != nz - 1) {
-            stnt(kd, hB + (d - 1) * S + s);
-            stnt(ke, hB + (e - 1) * S + s);
+            stnt(kd, hB + own + (d - 1) * nx);
+            stnt(ke, hB + own + (e - 1) * nx);
             if (z == nz - 2) {
-                stnt(kd, hB + (d - 1) * S + s + nxy);
-                stnt(ke, hB + (e - 1) * S + s + nxy);
+                stnt(kd, hB + own + ny * RW + (d - 1) * nx);
+                stnt(ke, hB + own + ny * RW + (e - 1) * nx);
             }
         }
     }

@@ -179,14 +179,16 @@ __global__ void __launch_bounds__(256) lbm_f64_kernel(LbmParams P, PopBufs pb, M
 {
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= P.S) return;
-    const unsigned nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy, S = P.S;
+    const unsigned nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy;
     const unsigned x = s % nx, t = s / nx, y = t % ny, z = t / ny;
     const unsigned xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
     const unsigned ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
     const unsigned zm = (z == 0) ? nz - 1 : z - 1, zp = (z == nz - 1) ? 0 : z + 1;
     const unsigned XS[3] = {xm, x, xp};
-    const unsigned YS[3] = {ym * nx, y * nx, yp * nx};
-    const unsigned ZS[3] = {zm * nxy, z * nxy, zp * nxy};
+    // populations in row blocks [z][y][d-1][x]; a pull along c reads row (y - c_y, z - c_z)
+    const unsigned RW = 26u * nx;
+    const unsigned YR[3] = {ym, y, yp}, ZR[3] = {zm * ny, z * ny, zp * ny};
+    const unsigned own = t * RW + x;
 
     const double* __restrict__ fA = (const double*)pb.f;
     const double* __restrict__ f0A = (const double*)pb.f0;
@@ -199,22 +201,22 @@ __global__ void __launch_bounds__(256) lbm_f64_kernel(LbmParams P, PopBufs pb, M
         ht[0] = h0A[s];
 #pragma unroll
         for (int d = 1; d < 27; ++d) {
-            ft[d] = fA[(d - 1) * S + s];
-            ht[d] = hA[(d - 1) * S + s];
+            ft[d] = fA[own + (d - 1) * nx];
+            ht[d] = hA[own + (d - 1) * nx];
         }
     } else {
 #pragma unroll
         for (int d = 1; d < 27; ++d)
-            ft[d] = fA[(d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]];
+            ft[d] = fA[(ZR[1 - CZ[d]] + YR[1 - CY[d]]) * RW + (d - 1) * nx + XS[1 - CX[d]]];
         if (z == 0) {
             ht[0] = -h0A[s] + P.inj[0];
 #pragma unroll
-            for (int d = 1; d < 27; ++d) ht[d] = -hA[(opp(d) - 1) * S + s] + P.inj[WCLS[d]];
+            for (int d = 1; d < 27; ++d) ht[d] = -hA[own + (opp(d) - 1) * nx] + P.inj[WCLS[d]];
         } else {
             ht[0] = h0A[s];
 #pragma unroll
             for (int d = 1; d < 27; ++d)
-                ht[d] = hA[(d - 1) * S + XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]];
+                ht[d] = hA[(ZR[1 - CZ[d]] + YR[1 - CY[d]]) * RW + (d - 1) * nx + XS[1 - CX[d]]];
         }
     }
 
@@ -249,18 +251,19 @@ __global__ void __launch_bounds__(256) lbm_f64_kernel(LbmParams P, PopBufs pb, M
         ftm[0] = 0.0;
         htm[0] = h0A[s1];
         if (FIRST) {
+            const unsigned own1 = own + ny * RW;
 #pragma unroll
             for (int d = 1; d < 27; ++d) {
-                ftm[d] = fA[(d - 1) * S + s1];
-                htm[d] = hA[(d - 1) * S + s1];
+                ftm[d] = fA[own1 + (d - 1) * nx];
+                htm[d] = hA[own1 + (d - 1) * nx];
             }
         } else {
-            const unsigned Z1[3] = {0u, nxy, 2u * nxy};
 #pragma unroll
             for (int d = 1; d < 27; ++d) {
-                const unsigned src = XS[1 - CX[d]] + YS[1 - CY[d]] + Z1[1 - CZ[d]];
-                ftm[d] = fA[(d - 1) * S + src];
-                htm[d] = hA[(d - 1) * S + src];
+                // node (x, y, 1) pulls from row (y - c_y, 1 - c_z)
+                const unsigned src = ((unsigned)(1 - CZ[d]) * ny + YR[1 - CY[d]]) * RW + (d - 1) * nx + XS[1 - CX[d]];
+                ftm[d] = fA[src];
+                htm[d] = hA[src];
             }
         }
         double Exm, Eym, Ezm;
@@ -300,7 +303,7 @@ __global__ void __launch_bounds__(256) lbm_f64_kernel(LbmParams P, PopBufs pb, M
     if (z == 0) {
         f0B[s] = ft[0];
 #pragma unroll
-        for (int d = 1; d < 27; ++d) fB[(d - 1) * S + s] = ft[opp(d)];
+        for (int d = 1; d < 27; ++d) fB[own + (d - 1) * nx] = ft[opp(d)];
     } else if (z == nz - 1) {
         f0B[s] = ft[0];
 #pragma unroll
@@ -308,23 +311,30 @@ __global__ void __launch_bounds__(256) lbm_f64_kernel(LbmParams P, PopBufs pb, M
             double v = ft[opp(d)];
             if (TOPSGN[d] > 0) v = v + P.multi[WCLS[d]];
             else if (TOPSGN[d] < 0) v = v - P.multi[WCLS[d]];
-            fB[(d - 1) * S + s] = v;
+            fB[own + (d - 1) * nx] = v;
         }
     } else {
         f0B[s] = fo[0];
 #pragma unroll
-        for (int d = 1; d < 27; ++d) fB[(d - 1) * S + s] = fo[d];
+        for (int d = 1; d < 27; ++d) fB[own + (d - 1) * nx] = fo[d];
     }
     if (z != nz - 1) {
         h0B[s] = ho[0];
 #pragma unroll
-        for (int d = 1; d < 27; ++d) hB[(d - 1) * S + s] = ho[d];
+        for (int d = 1; d < 27; ++d) hB[own + (d - 1) * nx] = ho[d];
         if (z == nz - 2) {
             h0B[s + nxy] = ho[0];
 #pragma unroll
-            for (int d = 1; d < 27; ++d) hB[(d - 1) * S + s + nxy] = ho[d];
+            for (int d = 1; d < 27; ++d) hB[own + ny * RW + (d - 1) * nx] = ho[d];
         }
     }
+}
+
+// Index of moving population d (1..26) of node `node` in the box's row-block
+// layout [z][y][d-1][x].
+__device__ __forceinline__ size_t pop_at(int d, unsigned node, unsigned nx)
+{
+    return (size_t)(node / nx) * 26u * nx + (size_t)(d - 1) * nx + node % nx;
 }
 
 // Equilibrium populations (gpu_init_equilibrium, LBM.cu:114-276) in FP64,
@@ -358,7 +368,6 @@ __global__ void __launch_bounds__(256) init_eq_kernel(LbmParams P, const double*
     const double rs = P.rs ? P.rs[z] : 0.0, qs = P.qs ? P.qs[z] : 0.0;
     f0[s] = (T)(wr[0] * (omusq) - P.w[0] * rs);
     h0[s] = (T)(wc[0] * (omusq_c) - P.w[0] * qs);
-    const unsigned S = P.S;
 #pragma unroll
     for (int d = 1; d < 27; ++d) {
         const int c = WCLS[d];
@@ -366,8 +375,8 @@ __global__ void __launch_bounds__(256) init_eq_kernel(LbmParams P, const double*
         const double cc = cdot<true>(d, tc);
         const double fe = wr[c] * (omusq + cu * (1.0 + 0.5 * cu));
         const double he = wc[c] * (omusq_c + cc * (1.0 + 0.5 * cc));
-        f[(d - 1) * S + s] = (T)(fe - P.w[c] * rs);
-        h[(d - 1) * S + s] = (T)(he - P.w[c] * qs);
+        f[pop_at(d, s, P.nx)] = (T)(fe - P.w[c] * rs);
+        h[pop_at(d, s, P.nx)] = (T)(he - P.w[c] * qs);
     }
 }
 
@@ -400,11 +409,11 @@ __global__ void __launch_bounds__(256) read_pre_kernel(LbmParams P, int direct, 
         const double wf = P.w[WCLS[d]];
         const unsigned src = direct ? s : (XS[1 - CX[d]] + YS[1 - CY[d]] + ZS[1 - CZ[d]]);
         const unsigned zs = direct ? z : ZP[1 - CZ[d]];
-        f1o[(d - 1) * S + s] = (double)f[(d - 1) * S + src] + wf * rsh(zs);
+        f1o[(d - 1) * S + s] = (double)f[pop_at(d, src, nx)] + wf * rsh(zs);
         if (!direct && z == 0)
-            h1o[(d - 1) * S + s] = -((double)h[(opp(d) - 1) * S + s] + wf * qsh(0)) + P.inj[WCLS[d]];
+            h1o[(d - 1) * S + s] = -((double)h[pop_at(opp(d), s, nx)] + wf * qsh(0)) + P.inj[WCLS[d]];
         else
-            h1o[(d - 1) * S + s] = (double)h[(d - 1) * S + src] + wf * qsh(zs);
+            h1o[(d - 1) * S + s] = (double)h[pop_at(d, src, nx)] + wf * qsh(zs);
     }
 }
 
@@ -423,8 +432,8 @@ __global__ void __launch_bounds__(256) write_pre_kernel(LbmParams P, const doubl
 #pragma unroll
     for (int d = 1; d < 27; ++d) {
         const double wf = P.w[WCLS[d]];
-        f[(d - 1) * S + s] = (T)(f1[(d - 1) * S + s] - wf * rs);
-        h[(d - 1) * S + s] = (T)(h1[(d - 1) * S + s] - wf * qs);
+        f[pop_at(d, s, P.nx)] = (T)(f1[(d - 1) * S + s] - wf * rs);
+        h[pop_at(d, s, P.nx)] = (T)(h1[(d - 1) * S + s] - wf * qs);
     }
 }
 
