@@ -497,14 +497,26 @@ __global__ void __launch_bounds__(256) ffty_kernel(const PoissonJob* __restrict_
 // thread loads its chunk straight into registers and solves with the
 // chunk's own matrix T (Thomas)
 //   T y = r,  T v = e_first,  (T w = e_last, w = reverse of v for Toeplitz T),
-// so that x = y - x_left v - x_right w.  The 2P interface unknowns (first and
-// last x of every chunk) satisfy a banded system that one thread per mode
-// eliminates in order, through per-chunk records in LDS (fields outermost,
-// modes innermost: the threads of a wave hit consecutive banks); the chunks
-// then finish independently and store from registers.  LDS: 6 P MB doubles.
-// (An earlier form staged the [n][MB] tile in LDS for the loads and stores:
-// the same arithmetic, bit-identical results, 31 us against 25 us here on
-// the 256x256x128 FP32 spectrum.)
+// so that x = y - x_left v - x_right w.  The interface unknowns (first and
+// last x of every chunk) obey three recurrences along the chunks:
+//   be_q = -vf_q - vl_q^2 be_{q-1} / (1 + vf_q be_{q-1})  (pivots),
+//   ar_q = al_q + bt_q ar_{q-1},   xf_q = gr_q + de_q xf_{q+1}
+// (vf, vl the first and last entries of the chunk's v), after which every
+// chunk has its neighbours x_l(q-1) = ar_{q-1} + be_{q-1} xf_q and x_f(q+1)
+// and finishes independently.  Per-chunk records go through LDS, fields
+// outermost and modes innermost, so the threads of a wave hit consecutive
+// banks.  Two ways to solve the recurrences:
+// - FP64 spectrum (the parity path): one thread per mode runs them in order.
+// - FP32 spectrum: three doubling (Hillis-Steele) scans over the P threads of
+//   the mode: prefix products of the Moebius matrices
+//   M_q = [[vl^2 - vf^2, -vf], [vf, 1]], be_q = N_q / D_q with
+//   [N_q; D_q] = M_q [N_{q-1}; D_{q-1}], rescaled by powers of two; an affine
+//   prefix for ar; an affine suffix for xf.  25 -> 21.5 us on the 256x256x128
+//   FP32 spectrum.  The composed matrices lose more to rounding than the
+//   ordered recurrence: against the oracle at 64x48x200 the FP64 solve was
+//   7e-13 with scans and 3e-14 in order, so the FP64 spectrum keeps the order.
+//   Rounded to float the two agree.
+// LDS: 6 P MB doubles.
 constexpr int ZS_MAXM = 16;
 
 template <typename T, int P, int MAXM, int NT = 256>
@@ -518,8 +530,8 @@ __global__ void __launch_bounds__(NT) zsolve_kernel(const PoissonJob* __restrict
     const int n = J.n;
     const double kxs = J.kxs, kys = J.kys, dz2 = J.dz2, voltage = J.voltage;
     extern __shared__ unsigned char smem_raw[];
-    double* xch = reinterpret_cast<double*>(smem_raw);       // [6][P][MB]
-    auto X = [&](int f, int q, int cc) -> double& { return xch[(f * P + q) * MB + cc]; };
+    double* sc = reinterpret_cast<double*>(smem_raw);   // [6][P][MB] chunk records
+    auto Y = [&](int f, int q, int cc) -> double& { return sc[(f * P + q) * MB + cc]; };
     const int m0 = blockIdx.x * MB;
     const int nm = min(MB, nmodes - m0);
     const int c = threadIdx.x % MB, p = threadIdx.x / MB;
@@ -542,6 +554,7 @@ __global__ void __launch_bounds__(NT) zsolve_kernel(const PoissonJob* __restrict
     const double kxv = kxi * kxs, kyv = (kyi <= ny / 2 ? kyi : kyi - ny) * kys;
     const double bd = -(2.0 + (kxv * kxv + kyv * kyv) * dz2);
     double cp[MAXM], yr[MAXM], yi[MAXM], v[MAXM];
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, vf = 0.0, vlq = 0.0;
     if (active) {
 #pragma unroll
         for (int i = 0; i < MAXM; ++i) {
@@ -562,62 +575,146 @@ __global__ void __launch_bounds__(NT) zsolve_kernel(const PoissonJob* __restrict
                 v[i] -= cp[i] * v[i + 1];
             }
         }
-        double ylr = 0.0, yli = 0.0, vl = 0.0;
 #pragma unroll
         for (int i = 0; i < MAXM; ++i)
             if (i == len - 1) {
-                ylr = yr[i];
-                yli = yi[i];
-                vl = v[i];
+                e2 = yr[i];
+                e3 = yi[i];
+                vlq = v[i];
             }
-        X(0, p, c) = yr[0]; X(1, p, c) = yi[0]; X(2, p, c) = ylr; X(3, p, c) = yli; X(4, p, c) = v[0];
-        X(5, p, c) = vl;
+        e0 = yr[0]; e1 = yi[0]; vf = v[0];
     }
-    __syncthreads();
-    if (p == 0 && c < nm) {
-        double ar = 0.0, ai = 0.0, be = 0.0;
+    double Lr = 0.0, Li = 0.0, Rr = 0.0, Ri = 0.0;
+    if constexpr (sizeof(T) == sizeof(float)) {
+        // (1) be_q = N_q / D_q, [N_q; D_q] = M_q [N_{q-1}; D_{q-1}], M_q = [[vl^2 - vf^2, -vf], [vf, 1]]:
+        //     inclusive prefix products of the M by a doubling scan (identity for inactive chunks)
+        double a = 1.0, b = 0.0, cc2 = 0.0, d = 1.0;
+        if (active) { a = vlq * vlq - vf * vf; b = -vf; cc2 = vf; d = 1.0; }
 #pragma unroll
-        for (int qq = 0; qq < P; ++qq) {
-            if (qq >= Peff) break;
-            const double e0 = X(0, qq, c), e1 = X(1, qq, c), e2 = X(2, qq, c), e3 = X(3, qq, c);
-            const double vf = X(4, qq, c), vlq = X(5, qq, c);
-            const double wf = vlq, wl = vf;
-            const double den = 1.0 / (1.0 + vf * be);
-            const double gr = (e0 - vf * ar) * den;
-            const double gi = (e1 - vf * ai) * den;
-            const double de = -wf * den;
-            const double nar = e2 - vlq * ar - vlq * be * gr;
-            const double nai = e3 - vlq * ai - vlq * be * gi;
-            const double nbe = -vlq * be * de - wl;
-            ar = nar; ai = nai; be = nbe;
-            X(0, qq, c) = gr; X(1, qq, c) = gi; X(2, qq, c) = de; X(3, qq, c) = ar; X(4, qq, c) = ai;
-            X(5, qq, c) = be;
-        }
-        // back: record q gets (x_l(q-1), x_f(q+1)) in fields 0..3, written
-        // after the record's own coefficients are read
-        double xfr = 0.0, xfi = 0.0;
-#pragma unroll
-        for (int qq = P - 1; qq >= 0; --qq) {
-            if (qq >= Peff) continue;
-            const double gr = X(0, qq, c), gi = X(1, qq, c), de = X(2, qq, c);
-            const double ar = X(3, qq, c), ai = X(4, qq, c), be = X(5, qq, c);
-            const double xlr = ar + be * xfr, xli = ai + be * xfi;
-            X(2, qq, c) = xfr;
-            X(3, qq, c) = xfi;
-            if (qq + 1 < Peff) {
-                X(0, qq + 1, c) = xlr;
-                X(1, qq + 1, c) = xli;
+        for (int off = 1; off < P; off <<= 1) {
+            Y(0, p, c) = a; Y(1, p, c) = b; Y(2, p, c) = cc2; Y(3, p, c) = d;
+            __syncthreads();
+            if (p >= off) {
+                const double a2 = Y(0, p - off, c), b2 = Y(1, p - off, c), c2 = Y(2, p - off, c), d2 = Y(3, p - off, c);
+                const double na = a * a2 + b * c2, nb = a * b2 + b * d2, nc = cc2 * a2 + d * c2, nd = cc2 * b2 + d * d2;
+                const double mx = fmax(fmax(fabs(na), fabs(nb)), fmax(fabs(nc), fabs(nd)));
+                const int e = __builtin_amdgcn_frexp_exp(mx);
+                a = ldexp(na, -e); b = ldexp(nb, -e); cc2 = ldexp(nc, -e); d = ldexp(nd, -e);
             }
-            const double nxr = gr + de * xfr, nxi = gi + de * xfi;
-            xfr = nxr;
-            xfi = nxi;
+            __syncthreads();
         }
-        X(0, 0, c) = 0.0;
-        X(1, 0, c) = 0.0;
+        // exclusive value (N_{p-1}, D_{p-1}) = P_{p-1} [0; 1]
+        Y(0, p, c) = b; Y(1, p, c) = d;
+        __syncthreads();
+        const double Nq = (p > 0) ? Y(0, p - 1, c) : 0.0, Dq = (p > 0) ? Y(1, p - 1, c) : 1.0;
+        __syncthreads();
+        double den = 0.0, de = 0.0, be = 0.0, al = 0.0, ali = 0.0, bt = 1.0;
+        if (active) {
+            const double inv = 1.0 / (Dq + vf * Nq);
+            den = Dq * inv;
+            const double bden = Nq * inv;
+            de = -vlq * den;
+            be = vlq * vlq * bden - vf;
+            al = e2 - vlq * bden * e0;
+            ali = e3 - vlq * bden * e1;
+            bt = -vlq * den;
+        } else {
+            bt = 1.0; al = 0.0; ali = 0.0;
+        }
+        // (2) ar_q = al_q + bt_q ar_{q-1}: doubling scan of affine maps (identity for inactive chunks)
+        double sa = al, sai = ali, sb = bt;
+#pragma unroll
+        for (int off = 1; off < P; off <<= 1) {
+            Y(0, p, c) = sa; Y(1, p, c) = sai; Y(2, p, c) = sb;
+            __syncthreads();
+            if (p >= off) {
+                const double pa = Y(0, p - off, c), pai = Y(1, p - off, c), pb = Y(2, p - off, c);
+                sa = sa + sb * pa;
+                sai = sai + sb * pai;
+                sb = sb * pb;
+            }
+            __syncthreads();
+        }
+        // sa = ar_p; exclusive ar_{p-1} from the left neighbour
+        Y(0, p, c) = sa; Y(1, p, c) = sai;
+        __syncthreads();
+        const double arp = (p > 0) ? Y(0, p - 1, c) : 0.0, aip = (p > 0) ? Y(1, p - 1, c) : 0.0;
+        __syncthreads();
+        const double gr = active ? (e0 - vf * arp) * den : 0.0, gi = active ? (e1 - vf * aip) * den : 0.0;
+        // (3) xf_q = gr_q + de_q xf_{q+1}, xf_Peff = 0: doubling scan from the right
+        double xa = gr, xai = gi, xb = active ? de : 1.0;
+#pragma unroll
+        for (int off = 1; off < P; off <<= 1) {
+            Y(0, p, c) = xa; Y(1, p, c) = xai; Y(2, p, c) = xb;
+            __syncthreads();
+            if (p + off < P) {
+                const double na = Y(0, p + off, c), nai = Y(1, p + off, c), nb = Y(2, p + off, c);
+                xa = xa + xb * na;
+                xai = xai + xb * nai;
+                xb = xb * nb;
+            }
+            __syncthreads();
+        }
+        // xa = x_f(p); neighbours: x_f(p+1) from the right, x_l(p-1) = ar_{p-1} + be_{p-1} x_f(p)
+        Y(0, p, c) = xa; Y(1, p, c) = xai; Y(2, p, c) = be;
+        __syncthreads();
+        Rr = (p + 1 < Peff) ? Y(0, p + 1, c) : 0.0;
+        Ri = (p + 1 < Peff) ? Y(1, p + 1, c) : 0.0;
+        const double bem = (p > 0) ? Y(2, p - 1, c) : 0.0;
+        Lr = (p > 0) ? arp + bem * xa : 0.0;
+        Li = (p > 0) ? aip + bem * xai : 0.0;
+    } else {
+        if (active) {
+            Y(0, p, c) = e0; Y(1, p, c) = e1; Y(2, p, c) = e2; Y(3, p, c) = e3; Y(4, p, c) = vf;
+            Y(5, p, c) = vlq;
+        }
+        __syncthreads();
+        if (p == 0 && c < nm) {
+            double ar = 0.0, ai = 0.0, be = 0.0;
+#pragma unroll
+            for (int qq = 0; qq < P; ++qq) {
+                if (qq >= Peff) break;
+                const double f0 = Y(0, qq, c), f1 = Y(1, qq, c), f2 = Y(2, qq, c), f3 = Y(3, qq, c);
+                const double wl = Y(4, qq, c), wf = Y(5, qq, c);   // w reversed v: w_l = v_f, w_f = v_l
+                const double dn = 1.0 / (1.0 + wl * be);
+                const double gr = (f0 - wl * ar) * dn;
+                const double gi = (f1 - wl * ai) * dn;
+                const double de = -wf * dn;
+                const double nar = f2 - wf * ar - wf * be * gr;
+                const double nai = f3 - wf * ai - wf * be * gi;
+                const double nbe = -wf * be * de - wl;
+                ar = nar; ai = nai; be = nbe;
+                Y(0, qq, c) = gr; Y(1, qq, c) = gi; Y(2, qq, c) = de; Y(3, qq, c) = ar; Y(4, qq, c) = ai;
+                Y(5, qq, c) = be;
+            }
+            // back: record q gets (x_l(q-1), x_f(q+1)) in fields 0..3, written
+            // after the record's own coefficients are read
+            double xfr = 0.0, xfi = 0.0;
+#pragma unroll
+            for (int qq = P - 1; qq >= 0; --qq) {
+                if (qq >= Peff) continue;
+                const double gr = Y(0, qq, c), gi = Y(1, qq, c), de = Y(2, qq, c);
+                const double ar = Y(3, qq, c), ai = Y(4, qq, c), be = Y(5, qq, c);
+                const double xlr = ar + be * xfr, xli = ai + be * xfi;
+                Y(2, qq, c) = xfr;
+                Y(3, qq, c) = xfi;
+                if (qq + 1 < Peff) {
+                    Y(0, qq + 1, c) = xlr;
+                    Y(1, qq + 1, c) = xli;
+                }
+                const double nxr = gr + de * xfr, nxi = gi + de * xfi;
+                xfr = nxr;
+                xfi = nxi;
+            }
+            Y(0, 0, c) = 0.0;
+            Y(1, 0, c) = 0.0;
+        }
+        __syncthreads();
+        if (active) {
+            Lr = Y(0, p, c); Li = Y(1, p, c); Rr = Y(2, p, c); Ri = Y(3, p, c);
+        }
     }
-    __syncthreads();
     if (active) {
-        const double Lr = X(0, p, c), Li = X(1, p, c), Rr = X(2, p, c), Ri = X(3, p, c);
         double wn = 0.0;
 #pragma unroll
         for (int i = MAXM - 1; i >= 0; --i) {
@@ -776,8 +873,9 @@ size_t zsolve_lds(int P) { return (size_t)6 * P * (256 / P) * sizeof(double); }
 int zsolve_partition(int n, size_t cs)
 {
     (void)cs;
-    // 16 threads per mode first (measured fastest at n = 126: 25 us against
-    // 42 us for 8 and 51 us for 32 on the 256x256 FP32 spectrum)
+    // 16 threads per mode first (measured fastest at n = 126 on the 256x256
+    // FP32 spectrum: 21.5 us with scans against 28 us for 32; in order 25 us
+    // against 42 us for 8 and 51 us for 32)
     for (int cand : {16, 8, 32})
         if ((n + cand - 1) / cand <= ZS_MAXM && zsolve_lds(cand) <= kMaxLds) return cand;
     return 0;
