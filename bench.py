@@ -43,7 +43,8 @@ def parse():
     ap.add_argument("--ny", type=int, default=256)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--precision", choices=("fp32s", "fp64"), default="fp32s")
-    ap.add_argument("--cpu-steps", type=int, default=8, help="oracle steps in the CPU sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=5,
+                    help="oracle steps per timed run of the CPU sample, three runs (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle OpenMP threads (0: OMP_NUM_THREADS, else the cores this process may use)")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch events (value only)")
@@ -63,6 +64,17 @@ def traffic_from_profiles(workload: str):
         return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(args, workload):
     from oracle import pyoracle as O
 
@@ -74,12 +86,18 @@ def cpu_baseline(args, workload):
     b.init_equilibrium()
     b.step(1)                       # warm-up step
     n = args.cpu_steps
-    t0 = time.perf_counter()
-    b.step(n)
-    dt = time.perf_counter() - t0
     S = args.nx * args.ny * args.nz
-    return {"value": round(S * n / dt / 1e6, 4), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, {workload}, {n} coupled steps after 1 warm-up step, {dt:.1f} s"}
+    rates, total = [], 0.0
+    for _ in range(3):              # median of three timed runs
+        t0 = time.perf_counter()
+        b.step(n)
+        dt = time.perf_counter() - t0
+        total += dt
+        rates.append(S * n / dt / 1e6)
+    return {"value": round(sorted(rates)[1], 4), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"FP64 oracle, {workload}, median of 3 runs of {n} coupled steps after 1 warm-up step, "
+                      f"{total:.1f} s in all"}
 
 
 def ensemble_params(E):

@@ -383,6 +383,35 @@ def test_fp32s_vs_fp64_developed_convection():
     assert max(errs["u"], errs["q"], errs["phi"]) < 2e-4, errs
 
 
+def test_fp32s_vs_fp64_256x256x128_20000_steps():
+    """The bench box's own start (256x256x128, T=400, C=10, hydrostatic q and
+    phi, 1e-3 velocity noise from seed 2) stepped 20,000 times in both
+    precisions from the same fields.  The velocity of this start oscillates
+    (umax swings between ~5e-2 and ~2e-5 over the run), so the relative error
+    of u is read at the end, step 20,000, where umax is ~3e-4.  Bar: u, q, phi
+    within 1e-5 rel-L2, Nu and current within 0.1 %."""
+    def params_of(prec, init=E.INIT_FIELDS):
+        return E.make_params(256, 256, 128, T=400.0, C_=10.0, lam=0.25, precision=prec, init_mode=init, seed=2,
+                             amp=1e-3)
+    with E.Box(params_of(E.FP64, E.INIT_HYDROSTATIC)) as b:
+        b.init()
+        start = _nine(b.read_fields())
+    with E.Box(params_of(E.FP64)) as r, E.Box(params_of(E.FP32S)) as a:
+        r.init(start)
+        a.init(start)
+        for k in range(5000, 20001, 5000):
+            r.step(5000)
+            a.step(5000)
+            print(f"step {k}", flush=True)
+        fr = r.read_fields(("q", "phi", "ux", "uy", "uz"))
+        fa = a.read_fields(("q", "phi", "ux", "uy", "uz"))
+        dr, da = r.diagnostics(), a.diagnostics()
+    errs = dict(_field_errors(fa, fr), Nu=abs(da.nusselt / dr.nusselt - 1), current=abs(da.current / dr.current - 1))
+    print(f"256x256x128 from noise, FP64 Nu {dr.nusselt:.6f} umax {dr.umax:.3e}; FP32S vs FP64 after 20000 steps: {errs}")
+    assert errs["u"] < 1e-5 and errs["q"] < 1e-5 and errs["phi"] < 1e-5, errs
+    assert errs["Nu"] < 1e-3 and errs["current"] < 1e-3, errs
+
+
 def test_fp32s_vs_fp64_256x256x128():
     """The bench box (256x256x128, T=400, C=10) started from hydrostatic q and
     phi with roll velocities of 20 % of the drift velocity: 1000 steps in both
