@@ -11,9 +11,9 @@ inverse FFTs).  MLUPS = nx*ny*nz*steps / (1e6 * seconds), walls included
 Prints one JSON line.  roofline: the fused LBM kernel (the dominant kernel),
 algorithmic bytes 440 B per node (2 x 27 populations read and written in
 FP32, q written, phi read) over its average launch time, taken from HIP
-events recorded on the box's stream around every launch inside the timed
-region.  cpu_baseline: the FP64 CPU oracle (OpenMP) on a bounded sample of
-the same workload.
+events recorded on the box's stream around the launch of every 8th step of
+the timed region (--profile-every).  cpu_baseline: the FP64 CPU oracle
+(OpenMP) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -43,11 +43,15 @@ def parse():
     ap.add_argument("--ny", type=int, default=256)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--precision", choices=("fp32s", "fp64"), default="fp32s")
+    ap.add_argument("--T", type=float, default=400.0, help="electric Rayleigh number of the box")
+    ap.add_argument("--seed", type=int, default=2, help="seed of the 1e-3 velocity noise")
     ap.add_argument("--cpu-steps", type=int, default=5,
                     help="oracle steps per timed run of the CPU sample, three runs (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle OpenMP threads (0: OMP_NUM_THREADS, else the cores this process may use)")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch events (value only)")
+    ap.add_argument("--profile-every", type=int, default=8,
+                    help="HIP events around the LBM and Poisson launches of every n-th step (each event record idles the stream for a few microseconds; every launch does the same work)")
     ap.add_argument("--workload", choices=("box", "ensemble"), default="box",
                     help="box: one 256x256x128 box (the headline line); ensemble: the 48-box sweep in one batch")
     return ap.parse_args()
@@ -80,9 +84,9 @@ def cpu_baseline(args, workload):
 
     threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     O.set_threads(threads)
-    p = O.make_params(args.nx, args.ny, args.nz, T=400.0, M=10.0, C_=10.0, Fe=4000.0, lam=0.25)
+    p = O.make_params(args.nx, args.ny, args.nz, T=args.T, M=10.0, C_=10.0, Fe=4000.0, lam=0.25)
     b = O.Box(p)
-    b.init_hydrostatic(2, 1e-3)
+    b.init_hydrostatic(args.seed, 1e-3)
     b.init_equilibrium()
     b.step(1)                       # warm-up step
     n = args.cpu_steps
@@ -170,14 +174,14 @@ def main():
     nx, ny, nz = args.nx, args.ny, args.nz
     S = nx * ny * nz
     workload = f"coupled EHD step {nx}x{ny}x{nz} {args.precision}"
-    params = E.make_params(nx, ny, nz, T=400.0, C_=10.0, M=10.0, Fe=4000.0, Pr=10.0, lam=0.25,
-                           precision=prec, init_mode=E.INIT_HYDROSTATIC, seed=2, amp=1e-3)
+    params = E.make_params(nx, ny, nz, T=args.T, C_=10.0, M=10.0, Fe=4000.0, Pr=10.0, lam=0.25,
+                           precision=prec, init_mode=E.INIT_HYDROSTATIC, seed=args.seed, amp=1e-3)
     box = E.Box(params)
     box.init()
     box.step(args.warmup)
     box.sync()
     if not args.no_profile:
-        box.profile_enable(True)
+        box.profile_enable(True, every=args.profile_every)
     box.profile_reset()
     t0 = time.perf_counter()
     box.step(args.steps)
@@ -194,8 +198,8 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if prec == E.FP32S else "f64", "data": "synthetic",
-        "config": {"workload": workload, "T": 400, "C": 10, "M": 10, "Fe": 4000, "Pr": 10,
-                   "lambda": 0.25, "init": "hydrostatic + 1e-3 noise, seed 2"},
+        "config": {"workload": workload, "T": args.T, "C": 10, "M": 10, "Fe": 4000, "Pr": 10,
+                   "lambda": 0.25, "init": f"hydrostatic + 1e-3 noise, seed {args.seed}"},
         "hbm_frac_step": round(STEP_BYTES_PER_NODE[prec] * S * args.steps / elapsed / 1e9 / HBM_PEAK_GBS, 4),
     }
     if not args.no_profile and prof.lbm_launches > 0:

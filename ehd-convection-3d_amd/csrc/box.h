@@ -53,7 +53,7 @@ struct ehd_box {
     double t = 0.0;
     double current_hs = 0.0;
     // profiling
-    bool prof = false;
+    int prof_every = 0;   // events around the launches of every prof_every-th step; 0: off
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> lbm_pairs, poi_pairs, step_pairs;
     double lbm_ms = 0.0, poi_ms = 0.0, step_ms = 0.0;

@@ -175,10 +175,14 @@ MacroBufs macro_bufs(ehd_box* b)
     return MacroBufs{b->rho, b->ux, b->uy, b->uz, b->q, b->phi, b->phibar, b->ex, b->ey, b->ez};
 }
 
+// launches of this step are bracketed by timing events
+bool profiled_step(const ehd_box* b) { return b->prof_every > 0 && b->step % b->prof_every == 0; }
+
 int run_lbm(ehd_box* b, bool macro, void* const* target_items)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (b->prof) {
+    const bool prof = profiled_step(b);
+    if (prof) {
         e0 = take_event(b);
         e1 = take_event(b);
         if (!e0 || !e1) return set_error(EHD_EHIP, "hipEventCreate failed");
@@ -190,7 +194,7 @@ int run_lbm(ehd_box* b, bool macro, void* const* target_items)
                                    (unsigned)((b->S + 255) / 256), 0, b->lbm_chunk, b->direct, b->ebuf, macro, b->st,
                                    target_items != nullptr);
     if (e != hipSuccess) return set_error(EHD_EHIP, "LBM launch failed: %s", hipGetErrorString(e));
-    if (b->prof) {
+    if (prof) {
         HIPCHK(hipEventRecord(e1, b->st));
         b->lbm_pairs.emplace_back(e0, e1);
     }
@@ -219,7 +223,8 @@ PoissonGroup poisson_group_of(const ehd_box* b)
 int run_poisson(ehd_box* b)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (b->prof) {
+    const bool prof = profiled_step(b);
+    if (prof) {
         e0 = take_event(b);
         e1 = take_event(b);
         if (!e0 || !e1) return set_error(EHD_EHIP, "hipEventCreate failed");
@@ -227,7 +232,7 @@ int run_poisson(ehd_box* b)
     }
     hipError_t e = launch_poisson_group(b->prec, poisson_group_of(b), b->st);
     if (e != hipSuccess) return set_error(EHD_EHIP, "Poisson launch failed: %s", hipGetErrorString(e));
-    if (b->prof) {
+    if (prof) {
         HIPCHK(hipEventRecord(e1, b->st));
         b->poi_pairs.emplace_back(e0, e1);
     }
@@ -648,7 +653,7 @@ int ehd_step(ehd_box* b, int nsteps)
     if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
     if (nsteps < 0) return set_error(EHD_EINVAL, "nsteps < 0");
     hipEvent_t s0 = nullptr, s1 = nullptr;
-    if (b->prof && nsteps > 0) {
+    if (b->prof_every > 0 && nsteps > 0) {
         s0 = take_event(b);
         s1 = take_event(b);
         if (!s0 || !s1) return set_error(EHD_EHIP, "hipEventCreate failed");
@@ -662,7 +667,7 @@ int ehd_step(ehd_box* b, int nsteps)
         b->step += 1;
         b->t = b->t + b->der.dt;   // main.cu:208
     }
-    if (b->prof && nsteps > 0) {
+    if (b->prof_every > 0 && nsteps > 0) {
         HIPCHK(hipEventRecord(s1, b->st));
         b->step_pairs.emplace_back(s0, s1);
         int rc = resolve_events(b);
@@ -979,7 +984,8 @@ int ehd_run_to_steady(ehd_box* b, int check_every, double tol, long long max_ste
 int ehd_profile_enable(ehd_box* b, int on)
 {
     if (!b) return set_error(EHD_EINVAL, "null box");
-    b->prof = (on != 0);
+    if (on < 0) return set_error(EHD_EINVAL, "profile sampling interval < 0 (got %d)", on);
+    b->prof_every = on;
     return EHD_OK;
 }
 

@@ -302,8 +302,9 @@ class Box:
     def set_nusselt_reference(self, current_hs: float):
         _check(lib().ehd_set_nusselt_reference(self._h, float(current_hs)))
 
-    def profile_enable(self, on: bool = True):
-        _check(lib().ehd_profile_enable(self._h, 1 if on else 0))
+    def profile_enable(self, on: bool = True, every: int = 1):
+        """Events around the LBM and Poisson launches of every ``every``-th step."""
+        _check(lib().ehd_profile_enable(self._h, int(every) if on else 0))
 
     def profile_read(self) -> Profile:
         pr = Profile()

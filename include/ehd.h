@@ -198,13 +198,17 @@ typedef struct ehd_steady {
 } ehd_steady;
 int ehd_run_to_steady(ehd_box* b, int check_every, double tol, long long max_steps, ehd_steady* out);
 
-/* Per-kernel timing with HIP events on the box's stream.  When enabled,
- * every launch of the fused LBM kernel and of the Poisson chain is bracketed
- * by events; ehd_profile_read returns accumulated milliseconds and counts. */
+/* Per-kernel timing with HIP events on the box's stream.  ehd_profile_enable
+ * (b, n): n = 0 disables; n >= 1 brackets the fused LBM launch and the
+ * Poisson chain of every step whose index is a multiple of n (n = 1: every
+ * step; every launch of either does the same work, and each event record
+ * idles the stream for a few microseconds, so a sample costs less), and
+ * every ehd_step call; ehd_profile_read returns accumulated milliseconds and
+ * counts. */
 typedef struct ehd_profile {
     double lbm_ms;      long long lbm_launches;
     double poisson_ms;  long long poisson_calls;
-    double step_ms;     long long steps;   /* wall time of ehd_step calls, event based */
+    double step_ms;     long long steps;   /* ehd_step calls: their event-timed wall time and their number */
 } ehd_profile;
 int ehd_profile_enable(ehd_box* b, int on);
 int ehd_profile_read(ehd_box* b, ehd_profile* out);

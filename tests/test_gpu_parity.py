@@ -256,6 +256,27 @@ def test_diagnostics_current_vs_oracle():
         assert d.step == 3
 
 
+def test_profile_sampling_counts_launches():
+    """ehd_profile_enable(b, n) brackets the LBM launch and the Poisson chain
+    of the steps whose index is a multiple of n, and every ehd_step call; the
+    sampled steps compute exactly what unsampled ones do."""
+    ob = oracle_box(16, 12, 10)
+    ob.init_equilibrium()
+    with gpu_box(ob) as b, gpu_box(ob) as ref:
+        b.profile_enable(True, every=4)
+        b.step(12)            # steps 0..11: 0, 4, 8 sampled
+        b.step(5)             # steps 12..16: 12, 16
+        pr = b.profile_read()
+        assert (pr.lbm_launches, pr.poisson_calls, pr.steps) == (5, 5, 2), pr
+        assert pr.lbm_ms > 0 and pr.poisson_ms > 0 and pr.step_ms > pr.lbm_ms
+        ref.step(17)
+        got, want = b.read_fields(("q", "uz")), ref.read_fields(("q", "uz"))
+        for k in ("q", "uz"):
+            assert_same(got[k], want[k], k)
+        with pytest.raises(E.EhdError):
+            b.profile_enable(True, every=-1)
+
+
 def test_nusselt_reference_and_steady_detector():
     """Nu = I / I_hs.  With the analytic reference Nu is 1 on the hydrostatic
     start.  The lattice's own no-flow steady current (a 2x2 column run to
