@@ -174,8 +174,12 @@ __device__ __forceinline__ void relax_f64(const LbmParams& P, const double* ft, 
     }
 }
 
+// Two waves per SIMD: left to itself the compiler gives the steady-state
+// variant 256 VGPRs + 4 AGPRs, one wave per SIMD; capped it keeps 256 VGPRs
+// and spills 5 dwords per lane to scratch.
 template <bool FIRST, bool EBUF>
-__global__ void __launch_bounds__(256) lbm_f64_kernel(LbmParams P, PopBufs pb, MacroBufs mb, int macro)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+lbm_f64_kernel(LbmParams P, PopBufs pb, MacroBufs mb, int macro)
 {
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= P.S) return;
