@@ -19,6 +19,8 @@ using namespace ehd;
 namespace {
 
 constexpr int kMaxGroups = 16;
+// streams the Poisson groups of a batch spread over, the batch stream included
+constexpr int kPoissonStreams = 4;
 
 struct Group {
     PoissonGroup g{};
@@ -49,6 +51,11 @@ struct ehd_batch {
     std::vector<double> group_ms;
     // graph of the steady step pair, per starting parity
     hipGraphExec_t exec[2] = {nullptr, nullptr};
+    // side streams (default priority) on which Poisson groups run beside the
+    // batch stream, and the events that fork them from it and join them back
+    std::vector<hipStream_t> side;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
 };
 
 namespace {
@@ -65,9 +72,28 @@ int batch_lbm(ehd_batch* bt, int cur, bool first, bool ebuf, bool macro)
     return EHD_OK;
 }
 
+// The groups touch disjoint boxes, so they run concurrently: group i on
+// stream i mod (1 + side streams), stream 0 being the batch stream.  The side
+// streams start after the work queued so far on the batch stream (the LBM
+// launch) and the batch stream continues after all of them; under graph
+// capture the same fork and join become the graph's branches.
 int batch_poisson(ehd_batch* bt)
 {
-    for (const Group& gr : bt->groups) HIPCHK(launch_poisson_group(bt->prec, gr.g, bt->st));
+    const size_t ns = bt->side.size();
+    if (ns == 0) {
+        for (const Group& gr : bt->groups) HIPCHK(launch_poisson_group(bt->prec, gr.g, bt->st));
+        return EHD_OK;
+    }
+    HIPCHK(hipEventRecord(bt->fork, bt->st));
+    for (hipStream_t s : bt->side) HIPCHK(hipStreamWaitEvent(s, bt->fork, 0));
+    for (size_t i = 0; i < bt->groups.size(); ++i) {
+        const size_t k = i % (ns + 1);
+        HIPCHK(launch_poisson_group(bt->prec, bt->groups[i].g, k == 0 ? bt->st : bt->side[k - 1]));
+    }
+    for (size_t k = 0; k < ns; ++k) {
+        HIPCHK(hipEventRecord(bt->join[k], bt->side[k]));
+        HIPCHK(hipStreamWaitEvent(bt->st, bt->join[k], 0));
+    }
     return EHD_OK;
 }
 
@@ -106,6 +132,15 @@ int capture_pair(ehd_batch* bt, int cur)
 void free_batch(ehd_batch* bt)
 {
     if (bt->st) (void)hipStreamSynchronize(bt->st);
+    for (hipStream_t s : bt->side) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+    bt->side.clear();
+    for (hipEvent_t e : bt->join) (void)hipEventDestroy(e);
+    bt->join.clear();
+    if (bt->fork) (void)hipEventDestroy(bt->fork);
+    bt->fork = nullptr;
     drop_graphs(bt);
     for (void* p : bt->ditems)
         if (p) (void)hipFree(p);
@@ -285,6 +320,23 @@ int ehd_batch_create(const ehd_params* params, int nboxes, int flags, ehd_batch*
         if (hipMemcpy(gr.djobs, jobs.data(), jobs.size() * sizeof(PoissonJob), hipMemcpyHostToDevice) != hipSuccess)
             return bail(set_error(EHD_EHIP, "job table upload failed"));
         gr.g.jobs = gr.djobs;
+    }
+    // up to kPoissonStreams - 1 side streams for the groups (with the batch
+    // stream that is the number of hardware queues a process gets by default)
+    const int nside = std::min((int)bt->groups.size(), kPoissonStreams) - 1;
+    if (nside > 0) {
+        if (hipEventCreateWithFlags(&bt->fork, hipEventDisableTiming) != hipSuccess)
+            return bail(set_error(EHD_EHIP, "hipEventCreate failed"));
+        for (int k = 0; k < nside; ++k) {
+            hipStream_t s = nullptr;
+            hipEvent_t e = nullptr;
+            if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+                return bail(set_error(EHD_EHIP, "hipStreamCreate failed"));
+            bt->side.push_back(s);
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                return bail(set_error(EHD_EHIP, "hipEventCreate failed"));
+            bt->join.push_back(e);
+        }
     }
     if (bt->prec == Prec::F32S) {
         const size_t isz = lbm_f32s_batch_item_size();
