@@ -26,6 +26,12 @@ struct Group {
     PoissonGroup g{};
     PoissonJob* djobs = nullptr;   // device table of the group's jobs
     std::vector<int> members;
+    // chains: the FP32S item tables of the members (tiles numbered from 0),
+    // their tile count and XCD order, and the stream the group runs on
+    void* items[2] = {nullptr, nullptr};
+    unsigned tiles = 0;
+    unsigned chunk = 0;
+    int stream = 0;                // 0: the batch stream, k: side stream k-1
 };
 
 PopBufs pop_bufs_at(ehd_box* b, int cur)
@@ -56,6 +62,10 @@ struct ehd_batch {
     std::vector<hipStream_t> side;
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
+    // one LBM launch and Poisson chain per group, each group on its stream,
+    // instead of one LBM launch for the whole batch
+    bool chains = false;
+    double step_ms[2] = {0.0, 0.0};   // timed step: one LBM launch / chains (tuning)
 };
 
 namespace {
@@ -77,24 +87,74 @@ int batch_lbm(ehd_batch* bt, int cur, bool first, bool ebuf, bool macro)
 // streams start after the work queued so far on the batch stream (the LBM
 // launch) and the batch stream continues after all of them; under graph
 // capture the same fork and join become the graph's branches.
-int batch_poisson(ehd_batch* bt)
+hipStream_t group_stream(const ehd_batch* bt, const Group& gr)
 {
-    const size_t ns = bt->side.size();
-    if (ns == 0) {
-        for (const Group& gr : bt->groups) HIPCHK(launch_poisson_group(bt->prec, gr.g, bt->st));
-        return EHD_OK;
-    }
+    return gr.stream == 0 ? bt->st : bt->side[gr.stream - 1];
+}
+
+// the side streams start after the work queued so far on the batch stream
+int fork_side(ehd_batch* bt)
+{
     HIPCHK(hipEventRecord(bt->fork, bt->st));
     for (hipStream_t s : bt->side) HIPCHK(hipStreamWaitEvent(s, bt->fork, 0));
-    for (size_t i = 0; i < bt->groups.size(); ++i) {
-        const size_t k = i % (ns + 1);
-        HIPCHK(launch_poisson_group(bt->prec, bt->groups[i].g, k == 0 ? bt->st : bt->side[k - 1]));
-    }
-    for (size_t k = 0; k < ns; ++k) {
+    return EHD_OK;
+}
+
+// the batch stream continues after all the side streams
+int join_side(ehd_batch* bt)
+{
+    for (size_t k = 0; k < bt->side.size(); ++k) {
         HIPCHK(hipEventRecord(bt->join[k], bt->side[k]));
         HIPCHK(hipStreamWaitEvent(bt->st, bt->join[k], 0));
     }
     return EHD_OK;
+}
+
+int batch_poisson(ehd_batch* bt)
+{
+    if (bt->side.empty()) {
+        for (const Group& gr : bt->groups) HIPCHK(launch_poisson_group(bt->prec, gr.g, bt->st));
+        return EHD_OK;
+    }
+    int rc = fork_side(bt);
+    if (rc) return rc;
+    for (const Group& gr : bt->groups) HIPCHK(launch_poisson_group(bt->prec, gr.g, group_stream(bt, gr)));
+    return join_side(bt);
+}
+
+// One step of every group on the group's own stream (chains), between a
+// fork_side and a join_side: a group's LBM launch waits only for its own
+// previous Poisson solve, so one group's Poisson passes overlap another's
+// LBM launch.
+int chain_step(ehd_batch* bt, int cur, bool first, bool ebuf, bool macro)
+{
+    for (const Group& gr : bt->groups) {
+        const hipStream_t s = group_stream(bt, gr);
+        if (bt->prec == Prec::F32S) {
+            HIPCHK(launch_lbm_f32s_batch(gr.items[cur], (int)gr.members.size(), gr.tiles, 0, gr.chunk, first, ebuf,
+                                         macro, s));
+        } else {
+            for (int m : gr.members) {
+                ehd_box* b = bt->boxes[m];
+                HIPCHK(launch_lbm_f64(b->lp, pop_bufs_at(b, cur), macro_bufs(b), first, ebuf, macro, s));
+            }
+        }
+        HIPCHK(launch_poisson_group(bt->prec, gr.g, s));
+    }
+    return EHD_OK;
+}
+
+// one coupled step of the whole batch, ending on the batch stream
+int batch_step_once(ehd_batch* bt, int cur, bool first, bool ebuf, bool macro)
+{
+    int rc;
+    if (!bt->chains) {
+        if ((rc = batch_lbm(bt, cur, first, ebuf, macro))) return rc;
+        return batch_poisson(bt);
+    }
+    if ((rc = fork_side(bt))) return rc;
+    if ((rc = chain_step(bt, cur, first, ebuf, macro))) return rc;
+    return join_side(bt);
 }
 
 void drop_graphs(ehd_batch* bt)
@@ -112,9 +172,16 @@ int capture_pair(ehd_batch* bt, int cur)
 {
     HIPCHK(hipStreamBeginCapture(bt->st, hipStreamCaptureModeThreadLocal));
     int rc = EHD_OK;
-    for (int k = 0; k < 2 && rc == EHD_OK; ++k) {
-        rc = batch_lbm(bt, (cur + k) & 1, false, false, false);
-        if (rc == EHD_OK) rc = batch_poisson(bt);
+    if (bt->chains) {
+        // the chains run both steps before they join
+        rc = fork_side(bt);
+        for (int k = 0; k < 2 && rc == EHD_OK; ++k) rc = chain_step(bt, (cur + k) & 1, false, false, false);
+        if (rc == EHD_OK) rc = join_side(bt);
+    } else {
+        for (int k = 0; k < 2 && rc == EHD_OK; ++k) {
+            rc = batch_lbm(bt, (cur + k) & 1, false, false, false);
+            if (rc == EHD_OK) rc = batch_poisson(bt);
+        }
     }
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(bt->st, &graph);
@@ -144,8 +211,11 @@ void free_batch(ehd_batch* bt)
     drop_graphs(bt);
     for (void* p : bt->ditems)
         if (p) (void)hipFree(p);
-    for (Group& g : bt->groups)
+    for (Group& g : bt->groups) {
         if (g.djobs) (void)hipFree(g.djobs);
+        for (void* p : g.items)
+            if (p) (void)hipFree(p);
+    }
     for (ehd_box* b : bt->boxes) ehd_destroy(b);
     if (bt->st) (void)hipStreamDestroy(bt->st);
 }
@@ -243,6 +313,15 @@ int tune(ehd_batch* bt)
             bt->boxes[m]->tune_ylines = by;
         }
     }
+    // one LBM launch for the whole batch, or one chain per group: a step of each
+    if (!bt->side.empty() && !(bt->flags & EHD_BATCH_CHAINS)) {
+        for (int mode = 0; mode < 2; ++mode) {
+            bt->chains = (mode == 1);
+            int rc = time_ms(bt, reps, [&] { return batch_step_once(bt, 0, false, false, false); }, &bt->step_ms[mode]);
+            if (rc) return rc;
+        }
+        bt->chains = bt->step_ms[1] < bt->step_ms[0];
+    }
     HIPCHK(hipStreamSynchronize(bt->st));
     return EHD_OK;
 }
@@ -337,6 +416,19 @@ int ehd_batch_create(const ehd_params* params, int nboxes, int flags, ehd_batch*
                 return bail(set_error(EHD_EHIP, "hipEventCreate failed"));
             bt->join.push_back(e);
         }
+        // groups to streams, largest first onto the least loaded stream (by nodes)
+        std::vector<size_t> order(bt->groups.size());
+        std::vector<double> nodes(bt->groups.size(), 0.0), load(nside + 1, 0.0);
+        for (size_t i = 0; i < order.size(); ++i) {
+            order[i] = i;
+            for (int m : bt->groups[i].members) nodes[i] += (double)bt->boxes[m]->S;
+        }
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return nodes[a] > nodes[b]; });
+        for (size_t i : order) {
+            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            bt->groups[i].stream = k;
+            load[k] += nodes[i];
+        }
     }
     if (bt->prec == Prec::F32S) {
         const size_t isz = lbm_f32s_batch_item_size();
@@ -358,7 +450,27 @@ int ehd_batch_create(const ehd_params* params, int nboxes, int flags, ehd_batch*
         bt->lbm_chunk = bt->boxes[0]->lbm_chunk;
         for (const ehd_box* b : bt->boxes)
             if (b->lbm_chunk != bt->lbm_chunk) bt->lbm_chunk = 0;
+        // per-group item tables for the chains (the boxes of a group share their plane size)
+        if (!bt->side.empty())
+            for (Group& gr : bt->groups) {
+                for (int cur = 0; cur < 2; ++cur) {
+                    std::vector<unsigned char> h(isz * gr.members.size());
+                    unsigned tile0 = 0;
+                    for (size_t j = 0; j < gr.members.size(); ++j) {
+                        ehd_box* b = bt->boxes[gr.members[j]];
+                        lbm_f32s_batch_item(h.data() + j * isz, b->lp, pop_bufs_at(b, cur), macro_bufs(b), tile0);
+                        tile0 += (unsigned)((b->S + 255) / 256);
+                    }
+                    gr.tiles = tile0;
+                    if (hipMalloc(&gr.items[cur], h.size()) != hipSuccess)
+                        return bail(set_error(EHD_ENOMEM, "item table allocation failed"));
+                    if (hipMemcpy(gr.items[cur], h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess)
+                        return bail(set_error(EHD_EHIP, "item table upload failed"));
+                }
+                gr.chunk = bt->boxes[gr.members[0]]->lbm_chunk;
+            }
     }
+    bt->chains = (flags & EHD_BATCH_CHAINS) != 0 && !bt->side.empty();
     if (flags & EHD_BATCH_TUNE) {
         int rc = tune(bt);
         if (rc) return bail(rc);
@@ -519,8 +631,7 @@ int ehd_batch_step(ehd_batch* bt, int nsteps)
             advance(2);
             continue;
         }
-        if ((rc = batch_lbm(bt, cur, direct, ebuf, last))) return rc;
-        if ((rc = batch_poisson(bt))) return rc;
+        if ((rc = batch_step_once(bt, cur, direct, ebuf, last))) return rc;
         cur = 1 - cur;
         direct = false;
         ebuf = false;
@@ -603,6 +714,9 @@ int ehd_batch_tuning(ehd_batch* bt, ehd_tuning* out)
     out->lbm_grid = (int)(bt->lbm_grid ? bt->lbm_grid : bt->total_tiles);
     out->lbm_ms = bt->lbm_ms;
     out->lbm_chunk = (int)bt->lbm_chunk;
+    out->chains = bt->chains ? 1 : 0;
+    out->step_ms[0] = bt->step_ms[0];
+    out->step_ms[1] = bt->step_ms[1];
     out->ngroups = (int)bt->groups.size();
     for (size_t i = 0; i < bt->groups.size(); ++i) {
         const Group& g = bt->groups[i];

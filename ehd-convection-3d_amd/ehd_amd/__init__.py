@@ -74,10 +74,11 @@ class _TuneGroup(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("lbm_grid", C.c_int), ("lbm_tiles", C.c_int), ("lbm_ms", C.c_double), ("ngroups", C.c_int),
-                ("group", _TuneGroup * 16), ("lbm_chunk", C.c_int)]
+                ("group", _TuneGroup * 16), ("lbm_chunk", C.c_int), ("chains", C.c_int),
+                ("step_ms", C.c_double * 2)]
 
 
-BATCH_GRAPH, BATCH_TUNE = 1, 2
+BATCH_GRAPH, BATCH_TUNE, BATCH_CHAINS = 1, 2, 4
 
 EXPORTS = (
     "ehd_params_default", "ehd_derive", "ehd_compute_parameters", "ehd_create", "ehd_destroy",

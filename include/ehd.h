@@ -231,6 +231,9 @@ typedef struct ehd_batch ehd_batch;
 
 #define EHD_BATCH_GRAPH 1   /* replay the steady steps from a captured HIP graph */
 #define EHD_BATCH_TUNE  2   /* time the launch shapes at creation and keep the fastest */
+#define EHD_BATCH_CHAINS 4  /* one LBM launch and Poisson chain per (nx, ny) group, the groups on
+                               concurrent streams (with EHD_BATCH_TUNE and without this flag the
+                               tuner times a step both ways and keeps the faster) */
 
 /* Launch shapes of a batch (the autotuner's choice or the defaults). */
 typedef struct ehd_tuning {
@@ -243,6 +246,8 @@ typedef struct ehd_tuning {
         double ms;                               /* timed Poisson chain of the group, 0 if not tuned */
     } group[16];
     int lbm_chunk;            /* XCD tile order of the LBM launch: tiles per XCD run (0: linear order) */
+    int chains;               /* 1: each group runs its own LBM launch and Poisson chain on its stream */
+    double step_ms[2];        /* timed step with one LBM launch / with chains (tuning), 0 if not timed */
 } ehd_tuning;
 
 /* nboxes parameter sets (all of one precision; at most 16 distinct (nx, ny)). */

@@ -34,12 +34,21 @@ def _assert_same_state(a, b, what):
 
 
 @pytest.mark.parametrize("precision,flags", [(E.FP32S, E.BATCH_GRAPH | E.BATCH_TUNE), (E.FP32S, 0),
-                                             (E.FP64, E.BATCH_GRAPH)])
+                                             (E.FP64, E.BATCH_GRAPH),
+                                             (E.FP32S, E.BATCH_GRAPH | E.BATCH_CHAINS), (E.FP32S, E.BATCH_CHAINS),
+                                             (E.FP64, E.BATCH_GRAPH | E.BATCH_CHAINS)])
 def test_batch_matches_single_boxes(precision, flags):
+    """Also with one LBM launch and Poisson chain per (nx, ny) group on
+    concurrent streams (BATCH_CHAINS), replayed from the graph or not."""
     params = _params(precision)
     nsteps = (1, 9, 14)            # first (direct) step, graph pairs, odd remainders
     with E.Batch(params, flags=flags) as bt:
-        print("tuning:", E_tuning_str(bt.tuning()))
+        t = bt.tuning()
+        print("tuning:", E_tuning_str(t))
+        if flags & E.BATCH_CHAINS:
+            assert t.chains == 1
+        elif not flags & E.BATCH_TUNE:
+            assert t.chains == 0
         bt.init()
         for n in nsteps:
             bt.step(n)
@@ -55,7 +64,8 @@ def test_batch_matches_single_boxes(precision, flags):
 def E_tuning_str(t):
     g = ", ".join(f"{t.group[i].nx}x{t.group[i].ny}: {t.group[i].nboxes} boxes x{t.group[i].x_lines}/y{t.group[i].y_lines}"
                   f" {t.group[i].ms:.3f} ms" for i in range(t.ngroups))
-    return f"lbm grid {t.lbm_grid} of {t.lbm_tiles} tiles ({t.lbm_ms:.3f} ms); {g}"
+    return (f"lbm grid {t.lbm_grid} of {t.lbm_tiles} tiles ({t.lbm_ms:.3f} ms); {g}; chains {t.chains} "
+            f"(step {t.step_ms[0]:.3f} ms one launch, {t.step_ms[1]:.3f} ms chains)")
 
 
 def test_batch_tuning_report_and_state_checks():
@@ -65,6 +75,9 @@ def test_batch_tuning_report_and_state_checks():
         assert t.lbm_tiles == sum((p.nx * p.ny * p.nz + 255) // 256 for p in params)
         assert 0 < t.lbm_grid <= t.lbm_tiles and t.lbm_ms > 0
         assert t.ngroups == 3 and sorted(t.group[i].nboxes for i in range(3)) == [1, 1, 2]
+        # three groups: the tuner timed a step with one LBM launch and with chains, kept the faster
+        assert t.step_ms[0] > 0 and t.step_ms[1] > 0
+        assert t.chains == (1 if t.step_ms[1] < t.step_ms[0] else 0)
         bt.init()
         bt.step(2)
         bt.boxes[1].step(1)            # one box stepped alone: the batch is out of step
