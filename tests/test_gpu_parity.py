@@ -433,6 +433,23 @@ def test_fp32s_vs_fp64_256x256x128_20000_steps():
     assert errs["Nu"] < 1e-3 and errs["current"] < 1e-3, errs
 
 
+def test_fp32s_vs_fp64_512x512x128():
+    """The largest configuration (512x512x128, 33.5 M nodes; T=1000, C=10,
+    hydrostatic start + 1e-3 noise from seed 5): 1000 steps in both
+    precisions from the same fields.  Bar: u, q, phi within 1e-5 rel-L2, Nu
+    and current within 0.1 %."""
+    def params_of(prec, init=E.INIT_FIELDS):
+        return E.make_params(512, 512, 128, T=1000.0, C_=10.0, lam=0.25, precision=prec, init_mode=init, seed=5,
+                             amp=1e-3)
+    with E.Box(params_of(E.FP64, E.INIT_HYDROSTATIC)) as b:
+        b.init()
+        start = _nine(b.read_fields())
+    errs, dr, _, _ = _fp32s_vs_fp64(params_of, start, 1000)
+    print(f"512x512x128, FP64 Nu {dr.nusselt:.6f}; FP32S vs FP64 after 1000 steps: {errs}")
+    assert errs["u"] < 1e-5 and errs["q"] < 1e-5 and errs["phi"] < 1e-5, errs
+    assert errs["Nu"] < 1e-3 and errs["current"] < 1e-3, errs
+
+
 def test_fp32s_vs_fp64_256x256x128():
     """The bench box (256x256x128, T=400, C=10) started from hydrostatic q and
     phi with roll velocities of 20 % of the drift velocity: 1000 steps in both
