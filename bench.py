@@ -152,7 +152,8 @@ def run_ensemble(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"48-box ensemble, {nodes} nodes, FP32S, one batch (graph + autotuned shapes)"},
         "one_box_at_a_time_MLUPS": round(nodes * nseq / seq / 1e6, 2),
-        "tuning": {"lbm_grid": tu.lbm_grid, "lbm_tiles": tu.lbm_tiles,
+        "tuning": {"lbm_grid": tu.lbm_grid, "lbm_tiles": tu.lbm_tiles, "chains": tu.chains,
+                   "step_ms_one_launch_vs_chains": [round(tu.step_ms[0], 4), round(tu.step_ms[1], 4)],
                    "groups": [[tu.group[i].nx, tu.group[i].nboxes, tu.group[i].x_lines, tu.group[i].y_lines]
                               for i in range(tu.ngroups)]},
     }

@@ -313,11 +313,25 @@ int tune(ehd_batch* bt)
             bt->boxes[m]->tune_ylines = by;
         }
     }
-    // one LBM launch for the whole batch, or one chain per group: a step of each
+    // one LBM launch for the whole batch, or one chain per group: a step of
+    // each, run as the batch will run it (the captured steady pair replayed,
+    // or eager)
     if (!bt->side.empty() && !(bt->flags & EHD_BATCH_CHAINS)) {
+        const bool graph = (bt->flags & EHD_BATCH_GRAPH) != 0;
         for (int mode = 0; mode < 2; ++mode) {
             bt->chains = (mode == 1);
-            int rc = time_ms(bt, reps, [&] { return batch_step_once(bt, 0, false, false, false); }, &bt->step_ms[mode]);
+            int rc;
+            if (graph) {
+                if ((rc = capture_pair(bt, 0))) return rc;
+                rc = time_ms(bt, reps, [&]() -> int {
+                    HIPCHK(hipGraphLaunch(bt->exec[0], bt->st));
+                    return EHD_OK;
+                }, &bt->step_ms[mode]);
+                drop_graphs(bt);
+                bt->step_ms[mode] *= 0.5;   // two steps per replay
+            } else {
+                rc = time_ms(bt, reps, [&] { return batch_step_once(bt, 0, false, false, false); }, &bt->step_ms[mode]);
+            }
             if (rc) return rc;
         }
         bt->chains = bt->step_ms[1] < bt->step_ms[0];

@@ -233,7 +233,8 @@ typedef struct ehd_batch ehd_batch;
 #define EHD_BATCH_TUNE  2   /* time the launch shapes at creation and keep the fastest */
 #define EHD_BATCH_CHAINS 4  /* one LBM launch and Poisson chain per (nx, ny) group, the groups on
                                concurrent streams (with EHD_BATCH_TUNE and without this flag the
-                               tuner times a step both ways and keeps the faster) */
+                               tuner times the steady step both ways, replayed from a graph when
+                               EHD_BATCH_GRAPH is set, and keeps the faster) */
 
 /* Launch shapes of a batch (the autotuner's choice or the defaults). */
 typedef struct ehd_tuning {
