@@ -17,3 +17,5 @@ for p in (ROOT, PKG):
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU (runs the HIP kernels)")
     config.addinivalue_line("markers", "slow: long-running parity case")
+    # per-test limit of pytest-timeout, above a run-wide --timeout for the long cases
+    config.addinivalue_line("markers", "timeout(seconds): per-test time limit")

@@ -404,6 +404,7 @@ def test_fp32s_vs_fp64_developed_convection():
     assert max(errs["u"], errs["q"], errs["phi"]) < 2e-4, errs
 
 
+@pytest.mark.timeout(300)   # ~45 s
 def test_fp32s_vs_fp64_256x256x128_20000_steps():
     """The bench box's own start (256x256x128, T=400, C=10, hydrostatic q and
     phi, 1e-3 velocity noise from seed 2) stepped 20,000 times in both

@@ -23,6 +23,7 @@ C = 10.0
 RUN = dict(check_every=2000, tol=1e-5, max_steps=300000)
 
 
+@pytest.mark.timeout(600)   # ~100 s: steady states of 8 boxes, 2000 training steps, warm runs
 def test_warm_start_gain():
     # training states and the cold runs of the held-out T, one batch
     draws = TRAIN_T + TEST_T
