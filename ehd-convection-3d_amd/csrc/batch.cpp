@@ -665,10 +665,19 @@ int ehd_batch_diagnostics(ehd_batch* bt, ehd_diag* out)
 {
     if (!bt || !out) return set_error(EHD_EINVAL, "null argument");
     HIPCHK(hipStreamSynchronize(bt->st));
+    int bad = 0;
+    long long nodes = 0;
     for (size_t i = 0; i < bt->boxes.size(); ++i) {
-        int rc = ehd_diagnostics(bt->boxes[i], &out[i]);
-        if (rc) return rc;
+        const int rc = ehd_diagnostics(bt->boxes[i], &out[i]);
+        if (rc == EHD_ENONFINITE) {
+            ++bad;
+            nodes += out[i].nonfinite;
+        } else if (rc) {
+            return rc;
+        }
     }
+    if (bad) return set_error(EHD_ENONFINITE, "watchdog: %d of %zu boxes non-finite (%lld nodes)", bad,
+                              bt->boxes.size(), nodes);
     return EHD_OK;
 }
 
@@ -678,26 +687,43 @@ int ehd_batch_run_to_steady(ehd_batch* bt, int check_every, double tol, long lon
     if (check_every < 1 || !(tol >= 0.0) || max_steps < 0) return set_error(EHD_EINVAL, "bad steady-state criteria");
     const size_t n = bt->boxes.size();
     std::vector<ehd_diag> d(n);
-    int rc = ehd_batch_diagnostics(bt, d.data());
+    // a box the watchdog flags is marked failed and left out of the test;
+    // the others go on (the boxes' buffers are disjoint)
+    auto diagnose = [&]() -> int {
+        const int r = ehd_batch_diagnostics(bt, d.data());
+        return (r == EHD_ENONFINITE) ? EHD_OK : r;
+    };
+    int rc = diagnose();
     if (rc) return rc;
     std::vector<double> prev(n);
+    size_t done = 0;
     for (size_t i = 0; i < n; ++i) {
         prev[i] = d[i].nusselt;
-        out[i] = ehd_steady{max_steps, 0, d[i].nusselt, d[i].current, 0.0};
+        out[i] = ehd_steady{max_steps, 0, 0, d[i].nusselt, d[i].current, 0.0};
+        if (d[i].nonfinite) {
+            out[i].failed = 1;
+            out[i].steps = 0;
+            ++done;
+        }
     }
     long long taken = 0;
-    size_t done = 0;
     while (taken < max_steps && done < n) {
         const int k = (int)std::min<long long>(check_every, max_steps - taken);
         if ((rc = ehd_batch_step(bt, k))) return rc;
         taken += k;
-        if ((rc = ehd_batch_diagnostics(bt, d.data()))) return rc;
+        if ((rc = diagnose())) return rc;
         for (size_t i = 0; i < n; ++i) {
-            const double delta = std::fabs(d[i].nusselt - prev[i]);
-            prev[i] = d[i].nusselt;
-            if (out[i].converged) continue;
+            if (out[i].converged || out[i].failed) continue;
             out[i].nusselt = d[i].nusselt;
             out[i].current = d[i].current;
+            if (d[i].nonfinite) {
+                out[i].failed = 1;
+                out[i].steps = taken;
+                ++done;
+                continue;
+            }
+            const double delta = std::fabs(d[i].nusselt - prev[i]);
+            prev[i] = d[i].nusselt;
             out[i].delta = delta;
             if (k == check_every && delta < tol) {
                 out[i].converged = 1;
@@ -707,7 +733,7 @@ int ehd_batch_run_to_steady(ehd_batch* bt, int check_every, double tol, long lon
         }
     }
     for (size_t i = 0; i < n; ++i)
-        if (!out[i].converged) out[i].steps = taken;
+        if (!out[i].converged && !out[i].failed) out[i].steps = taken;
     return EHD_OK;
 }
 

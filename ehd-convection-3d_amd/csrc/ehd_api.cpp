@@ -340,7 +340,34 @@ struct CkptHeader {
     long long step;
     double t;
     double T, C, M, Fe, lambda_f, lambda_c;
+    double cfl, voltage, eps, rho0, lz, uw, exf;
 };
+
+void ckpt_physics(const ehd_params& p, CkptHeader& h)
+{
+    h.T = p.T; h.C = p.C; h.M = p.M; h.Fe = p.Fe;
+    h.lambda_f = p.lambda_f; h.lambda_c = p.lambda_c;
+    h.cfl = p.cfl; h.voltage = p.voltage; h.eps = p.eps; h.rho0 = p.rho0; h.lz = p.lz;
+    h.uw = p.uw; h.exf = p.exf;
+}
+
+// first physical parameter in which a checkpoint differs from the box, or null
+const char* ckpt_mismatch(const CkptHeader& h, const ehd_params& p)
+{
+    CkptHeader w{};
+    ckpt_physics(p, w);
+    const struct {
+        const char* name;
+        double a, b;
+    } f[] = {{"T", h.T, w.T},       {"C", h.C, w.C},           {"M", h.M, w.M},
+             {"Fe", h.Fe, w.Fe},    {"lambda_f", h.lambda_f, w.lambda_f},
+             {"lambda_c", h.lambda_c, w.lambda_c}, {"cfl", h.cfl, w.cfl}, {"voltage", h.voltage, w.voltage},
+             {"eps", h.eps, w.eps}, {"rho0", h.rho0, w.rho0},  {"lz", h.lz, w.lz},
+             {"uw", h.uw, w.uw},    {"exf", h.exf, w.exf}};
+    for (const auto& x : f)
+        if (!(x.a == x.b)) return x.name;
+    return nullptr;
+}
 
 int ckpt_io(ehd_box* b, FILE* f, bool save)
 {
@@ -844,7 +871,7 @@ int ehd_save_state(ehd_box* b, const char* path)
     FILE* f = std::fopen(path, "wb");
     if (!f) return set_error(EHD_EINVAL, "cannot open %s for writing", path);
     CkptHeader h{};
-    std::memcpy(h.magic, "EHDCKPT1", 8);
+    std::memcpy(h.magic, "EHDCKPT2", 8);
     h.abi = EHD_ABI_VERSION;
     h.precision = b->prm.precision;
     h.nx = b->nx; h.ny = b->ny; h.nz = b->nz;
@@ -852,8 +879,7 @@ int ehd_save_state(ehd_box* b, const char* path)
     h.ebuf = b->ebuf ? 1 : 0;
     h.step = b->step;
     h.t = b->t;
-    h.T = b->prm.T; h.C = b->prm.C; h.M = b->prm.M; h.Fe = b->prm.Fe;
-    h.lambda_f = b->prm.lambda_f; h.lambda_c = b->prm.lambda_c;
+    ckpt_physics(b->prm, h);
     int rc = (std::fwrite(&h, sizeof(h), 1, f) == 1) ? ckpt_io(b, f, true) : set_error(EHD_EINVAL, "checkpoint write failed");
     if (std::fclose(f) != 0 && rc == EHD_OK) rc = set_error(EHD_EINVAL, "checkpoint close failed");
     return rc;
@@ -865,14 +891,22 @@ int ehd_load_state(ehd_box* b, const char* path)
     FILE* f = std::fopen(path, "rb");
     if (!f) return set_error(EHD_EINVAL, "cannot open %s", path);
     CkptHeader h{};
-    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, "EHDCKPT1", 8) != 0) {
+    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, "EHDCKPT2", 8) != 0) {
         std::fclose(f);
-        return set_error(EHD_EINVAL, "%s is not a checkpoint", path);
+        return set_error(EHD_EINVAL, "%s is not a checkpoint of this format", path);
+    }
+    if (h.abi != EHD_ABI_VERSION) {
+        std::fclose(f);
+        return set_error(EHD_EINVAL, "checkpoint of ABI version %d, library %d", h.abi, EHD_ABI_VERSION);
     }
     if (h.precision != b->prm.precision || h.nx != b->nx || h.ny != b->ny || h.nz != b->nz) {
         std::fclose(f);
         return set_error(EHD_EINVAL, "checkpoint grid/precision (%dx%dx%d, %d) differ from the box", h.nx, h.ny, h.nz,
                     h.precision);
+    }
+    if (const char* what = ckpt_mismatch(h, b->prm)) {
+        std::fclose(f);
+        return set_error(EHD_EINVAL, "checkpoint parameter %s differs from the box's", what);
     }
     HIPCHK(hipStreamSynchronize(b->st));
     const int rc = ckpt_io(b, f, false);
@@ -897,7 +931,8 @@ int ehd_diagnostics(ehd_box* b, ehd_diag* out)
 {
     if (!b || !out) return set_error(EHD_EINVAL, "null argument");
     if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
-    HIPCHK(launch_diag(b->prec, b->lp, b->q, b->uz, b->rho, b->phi, b->phibar, b->diag_scr, b->diag_out, b->st));
+    HIPCHK(launch_diag(b->prec, b->lp, b->q, b->ux, b->uy, b->uz, b->rho, b->phi, b->phibar, b->diag_scr,
+                       b->diag_out, b->st));
     double r[8] = {0};
     HIPCHK(hipMemcpyAsync(r, b->diag_out, 5 * sizeof(double), hipMemcpyDeviceToHost, b->st));
     HIPCHK(hipStreamSynchronize(b->st));
@@ -909,7 +944,10 @@ int ehd_diagnostics(ehd_box* b, ehd_diag* out)
     out->nusselt = (b->current_hs != 0.0) ? out->current / b->current_hs : 0.0;
     out->step = b->step;
     out->time = b->t;
-    if (r[4] != 0.0) return set_error(EHD_ENONFINITE, "non-finite charge or velocity at %.0f nodes (step %lld)", r[4], b->step);
+    out->nonfinite = (long long)r[4];
+    if (r[4] != 0.0)
+        return set_error(EHD_ENONFINITE, "non-finite charge, density or velocity at %.0f nodes (step %lld)", r[4],
+                         b->step);
     return EHD_OK;
 }
 
@@ -958,16 +996,30 @@ int ehd_run_to_steady(ehd_box* b, int check_every, double tol, long long max_ste
     if (check_every < 1 || !(tol >= 0.0) || max_steps < 0) return set_error(EHD_EINVAL, "bad steady-state criteria");
     if (!b->initialised) return set_error(EHD_ESTATE, "box not initialised");
     ehd_diag dg;
+    out->steps = 0;
+    out->converged = 0;
+    out->failed = 0;
+    out->delta = 0.0;
     int rc = ehd_diagnostics(b, &dg);
+    out->nusselt = dg.nusselt;
+    out->current = dg.current;
+    if (rc == EHD_ENONFINITE) out->failed = 1;
     if (rc) return rc;
     double prev = dg.nusselt;
     long long taken = 0;
-    out->converged = 0;
     while (taken < max_steps) {
         const int n = (int)std::min<long long>(check_every, max_steps - taken);
         if ((rc = ehd_step(b, n))) return rc;
         taken += n;
-        if ((rc = ehd_diagnostics(b, &dg))) return rc;
+        if ((rc = ehd_diagnostics(b, &dg))) {
+            if (rc == EHD_ENONFINITE) {
+                out->failed = 1;
+                out->steps = taken;
+                out->nusselt = dg.nusselt;
+                out->current = dg.current;
+            }
+            return rc;
+        }
         out->delta = std::fabs(dg.nusselt - prev);
         prev = dg.nusselt;
         if (n == check_every && out->delta < tol) {

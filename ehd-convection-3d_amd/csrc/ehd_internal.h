@@ -136,9 +136,10 @@ hipError_t launch_poisson_group(Prec prec, const PoissonGroup& g, hipStream_t st
 
 // --- diagnostics (diag.hip) --------------------------------------------------
 // out[0]=sum_xy q_top*Ez_top, out[1]=sum interior q, out[2]=max uz (top extrapolated),
-// out[3]=sum interior rho, out[4]=nonfinite count
-hipError_t launch_diag(Prec prec, const LbmParams& P, const void* q, const void* uz, const void* rho,
-                       const void* phi, const double* phibar, double* scratch, double* out,
+// out[3]=sum interior rho, out[4]=count of nodes with a non-finite q, rho, ux, uy or uz
+// scratch: 5 * diag_blocks() doubles of partial results
+hipError_t launch_diag(Prec prec, const LbmParams& P, const void* q, const void* ux, const void* uy, const void* uz,
+                       const void* rho, const void* phi, const double* phibar, double* scratch, double* out,
                        hipStream_t st);
 
 }  // namespace ehd

@@ -169,8 +169,9 @@ __global__ void __launch_bounds__(256) phi_split_kernel(LbmParams P, const doubl
 constexpr int DIAG_N = 5;
 
 template <typename T>
-__global__ void __launch_bounds__(256) diag_partial_kernel(LbmParams P, const T* q, const T* uz, const T* rho,
-                                                           const T* phi, const double* phibar, double* part)
+__global__ void __launch_bounds__(256) diag_partial_kernel(LbmParams P, const T* q, const T* ux, const T* uy,
+                                                           const T* uz, const T* rho, const T* phi,
+                                                           const double* phibar, double* part)
 {
     __shared__ double red[DIAG_N][256];
     const unsigned nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy;
@@ -181,10 +182,16 @@ __global__ void __launch_bounds__(256) diag_partial_kernel(LbmParams P, const T*
         const unsigned z = (unsigned)(s / nxy);
         const double qv = (double)q[s];
         const double uv = (double)uz[s];
-        if (!(qv == qv) || !(uv == uv) || fabs(qv) > 1.0e300 || fabs(uv) > 1.0e300) bad += 1.0;
+        const double rv = (double)rho[s];
+        // watchdog: a node counts once if any of its fields is NaN or infinite
+        // (|v| <= DBL_MAX is false for both)
+        const double big = 1.7976931348623157e308;
+        if (!(fabs(qv) <= big) || !(fabs(uv) <= big) || !(fabs(rv) <= big) || !(fabs((double)ux[s]) <= big) ||
+            !(fabs((double)uy[s]) <= big))
+            bad += 1.0;
         if (z >= 1 && z <= nz - 2) {
             qs += qv;
-            rs += (double)rho[s];
+            rs += rv;
         }
         double uvv = uv;
         if (z == nz - 1) uvv = 2.0 * (double)uz[s - nxy] - (double)uz[s - 2 * nxy];
@@ -298,16 +305,19 @@ hipError_t launch_phi_split(Prec prec, const LbmParams& P, const double* phi, vo
 
 int diag_blocks() { return 512; }
 
-hipError_t launch_diag(Prec prec, const LbmParams& P, const void* q, const void* uz, const void* rho,
-                       const void* phi, const double* phibar, double* scratch, double* out, hipStream_t st)
+hipError_t launch_diag(Prec prec, const LbmParams& P, const void* q, const void* ux, const void* uy, const void* uz,
+                       const void* rho, const void* phi, const double* phibar, double* scratch, double* out,
+                       hipStream_t st)
 {
     const int nb = diag_blocks();
     if (prec == Prec::F64)
-        diag_partial_kernel<double><<<nb, 256, 0, st>>>(P, (const double*)q, (const double*)uz, (const double*)rho,
-                                                        (const double*)phi, nullptr, scratch);
+        diag_partial_kernel<double><<<nb, 256, 0, st>>>(P, (const double*)q, (const double*)ux, (const double*)uy,
+                                                        (const double*)uz, (const double*)rho, (const double*)phi,
+                                                        nullptr, scratch);
     else
-        diag_partial_kernel<float><<<nb, 256, 0, st>>>(P, (const float*)q, (const float*)uz, (const float*)rho,
-                                                       (const float*)phi, phibar, scratch);
+        diag_partial_kernel<float><<<nb, 256, 0, st>>>(P, (const float*)q, (const float*)ux, (const float*)uy,
+                                                       (const float*)uz, (const float*)rho, (const float*)phi, phibar,
+                                                       scratch);
     diag_final_kernel<<<1, 256, 0, st>>>(scratch, nb, out);
     return hipGetLastError();
 }

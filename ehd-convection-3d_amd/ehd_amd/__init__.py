@@ -53,7 +53,7 @@ class Derived(C.Structure):
 class Diag(C.Structure):
     _fields_ = [("current", C.c_double), ("total_charge", C.c_double), ("umax", C.c_double),
                 ("nusselt", C.c_double), ("mean_rho", C.c_double), ("step", C.c_longlong),
-                ("time", C.c_double)]
+                ("time", C.c_double), ("nonfinite", C.c_longlong)]
 
 
 class Profile(C.Structure):
@@ -63,7 +63,7 @@ class Profile(C.Structure):
 
 
 class Steady(C.Structure):
-    _fields_ = [("steps", C.c_longlong), ("converged", C.c_int), ("nusselt", C.c_double),
+    _fields_ = [("steps", C.c_longlong), ("converged", C.c_int), ("failed", C.c_int), ("nusselt", C.c_double),
                 ("current", C.c_double), ("delta", C.c_double)]
 
 
@@ -89,7 +89,7 @@ EXPORTS = (
     "ehd_hydrostatic_current", "ehd_run_to_steady", "ehd_hydrostatic_reference",
     "ehd_init_consistent", "ehd_batch_init_consistent",
     "ehd_batch_create", "ehd_batch_destroy", "ehd_batch_init", "ehd_batch_step", "ehd_batch_diagnostics",
-    "ehd_batch_run_to_steady", "ehd_batch_box", "ehd_batch_tuning",
+    "ehd_batch_run_to_steady", "ehd_batch_box", "ehd_batch_tuning", "ehd_copy_ceiling",
     # host-only file layer, include/ehd_io.h
     "ehd_io_tecplot", "ehd_io_data_end", "ehd_io_read_end", "ehd_io_dmd", "ehd_io_umax", "ehd_io_current",
 )
@@ -155,6 +155,7 @@ def lib() -> C.CDLL:
     L.ehd_last_error.argtypes = []
     L.ehd_last_error.restype = C.c_char_p
     L.ehd_abi_version.argtypes = []
+    L.ehd_copy_ceiling.argtypes = [C.c_size_t, C.c_int, dp]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("ehd_destroy", "ehd_last_error", "ehd_io_current", "ehd_batch_destroy", "ehd_batch_box"):
@@ -201,6 +202,14 @@ def hydrostatic_current(p: Params) -> float:
     v = C.c_double()
     _check(lib().ehd_hydrostatic_current(C.byref(p), C.byref(v)))
     return v.value
+
+
+def copy_ceiling(nbytes: int = 1 << 31, reps: int = 20):
+    """GB/s of a plain and a nontemporal float4 device-to-device copy of
+    nbytes (read + written), the practical HBM ceiling of this GPU."""
+    out = np.zeros(2)
+    _check(lib().ehd_copy_ceiling(int(nbytes), int(reps), _dp(out)))
+    return float(out[0]), float(out[1])
 
 
 def derive(p: Params) -> Derived:
@@ -295,9 +304,13 @@ class Box:
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (f0, f1, h0, h1)]
         _check(lib().ehd_write_populations(self._h, *[_dp(a) for a in arrs]))
 
-    def diagnostics(self) -> Diag:
+    def diagnostics(self, allow_nonfinite: bool = False) -> Diag:
+        """FP64 reductions; the watchdog's EHD_ENONFINITE raises unless
+        allow_nonfinite (then d.nonfinite > 0 tells)."""
         d = Diag()
-        _check(lib().ehd_diagnostics(self._h, C.byref(d)))
+        rc = lib().ehd_diagnostics(self._h, C.byref(d))
+        if not (allow_nonfinite and rc == -5):
+            _check(rc)
         return d
 
     def set_nusselt_reference(self, current_hs: float):
@@ -433,9 +446,13 @@ class Batch:
     def step(self, n: int = 1):
         _check(lib().ehd_batch_step(self._h, int(n)))
 
-    def diagnostics(self):
+    def diagnostics(self, allow_nonfinite: bool = False):
+        """Diagnostics of every box; with allow_nonfinite the boxes the
+        watchdog flags come back with nonfinite > 0 instead of raising."""
         out = (Diag * len(self.boxes))()
-        _check(lib().ehd_batch_diagnostics(self._h, out))
+        rc = lib().ehd_batch_diagnostics(self._h, out)
+        if not (allow_nonfinite and rc == -5):
+            _check(rc)
         return list(out)
 
     def run_to_steady(self, check_every: int = 1000, tol: float = 1e-6, max_steps: int = 10**6):

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define EHD_ABI_VERSION 1
+#define EHD_ABI_VERSION 2
 
 /* error codes */
 #define EHD_OK             0
@@ -87,6 +87,8 @@ typedef struct ehd_diag {
     double mean_rho;      /* mean density over interior nodes */
     long long step;       /* steps taken since init */
     double time;          /* physical time t (main.cu:208) */
+    long long nonfinite;  /* nodes whose q, rho, ux, uy or uz is NaN or infinite (the watchdog;
+                             ehd_diagnostics then returns EHD_ENONFINITE, with every field filled) */
 } ehd_diag;
 
 typedef struct ehd_box ehd_box;
@@ -161,7 +163,10 @@ int ehd_write_populations(ehd_box* b, const double* f0, const double* f1,
  * restart file data_end.dat (LBM.cu:1505-1566, read back by read_data,
  * LBM.cu:1596-1643) keeps 6 decimals of the macroscopic fields and
  * re-equilibrates the populations; this one restarts bit for bit.  Loading
- * needs a box created with the same grid and precision. */
+ * needs a box created with the same ABI version, grid, precision and physics
+ * (T, C, M, Fe, both magic parameters, CFL, voltage, eps, rho0, Lz, uw, exf:
+ * FP32S populations are stored against hydrostatic tables built from them);
+ * any difference is EHD_EINVAL and the box is left unchanged. */
 int ehd_save_state(ehd_box* b, const char* path);
 int ehd_load_state(ehd_box* b, const char* path);
 
@@ -190,10 +195,13 @@ int ehd_hydrostatic_reference(ehd_box* b, int check_every, double tol, long long
 
 /* Steady-state detector: steps the box in chunks of check_every steps and
  * stops once |Nu(after chunk) - Nu(before chunk)| < tol (e.g. 1e-6 over 1000
- * steps), or after max_steps.  steps counts the steps taken by this call. */
+ * steps), or after max_steps.  steps counts the steps taken by this call.
+ * A box the watchdog finds non-finite stops with failed = 1 and the call
+ * returns EHD_ENONFINITE (out filled). */
 typedef struct ehd_steady {
     long long steps;
     int converged;
+    int failed;           /* the watchdog found non-finite fields (steps: the step of that check) */
     double nusselt, current, delta;
 } ehd_steady;
 int ehd_run_to_steady(ehd_box* b, int check_every, double tol, long long max_steps, ehd_steady* out);
@@ -216,6 +224,16 @@ int ehd_profile_reset(ehd_box* b);
 
 int ehd_sync(ehd_box* b);
 const char* ehd_last_error(void);
+
+/* Practical HBM ceiling of the current GPU (no reference counterpart; the
+ * reference computes a bandwidth figure at main.cu:251-260 but has no
+ * ceiling to hold it against): a float4 device-to-device copy of `bytes`
+ * bytes (bytes read and bytes written per launch), plain and with
+ * nontemporal loads and stores, timed with HIP events over `reps` launches
+ * after three warm-up launches.  out2[0] / out2[1]: GB/s (2 * bytes per
+ * launch) of the plain / nontemporal copy.  Allocates two buffers of
+ * `bytes` for the duration of the call. */
+int ehd_copy_ceiling(size_t bytes, int reps, double* out2);
 
 /* ---- batches: many independent boxes stepped together ------------------
  * Stability maps and hysteresis sweeps run many small boxes of uneven size
@@ -261,11 +279,15 @@ int ehd_batch_step(ehd_batch* bt, int nsteps);
  * as ehd_init does and step normally meanwhile); every box advances
  * `iterations` steps.  FP32S batches. */
 int ehd_batch_init_consistent(ehd_batch* bt, const double* const* fields9, int iterations);
-/* ehd_diagnostics of every box (out: nboxes entries) */
+/* ehd_diagnostics of every box (out: nboxes entries, all filled); returns
+ * EHD_ENONFINITE if the watchdog flagged any box (out[i].nonfinite > 0). */
 int ehd_batch_diagnostics(ehd_batch* bt, ehd_diag* out);
 /* steps the batch in chunks of check_every until every box has |dNu| < tol
  * over a chunk (or max_steps); out[i].steps is the step count at which box i
- * first met the criterion (max_steps if never). */
+ * first met the criterion (max_steps if never).  A box the watchdog flags
+ * gets failed = 1 (steps: the check that found it) and is no longer tested;
+ * the batch keeps stepping the others (a box's non-finite values stay in its
+ * own buffers), and the call returns EHD_OK. */
 int ehd_batch_run_to_steady(ehd_batch* bt, int check_every, double tol, long long max_steps, ehd_steady* out);
 /* member box i (owned by the batch; valid until ehd_batch_destroy) for the
  * per-box calls: fields, populations, checkpoints, Nusselt reference. */

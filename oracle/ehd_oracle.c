@@ -694,6 +694,12 @@ void oref_rates(const oref_box* b, double* out4)
     out4[2] = b->omega_c_plus; out4[3] = b->omega_c_minus;
 }
 
+void oref_set_rates(oref_box* b, const double* in4)
+{
+    b->omega_plus = in4[0]; b->omega_minus = in4[1];
+    b->omega_c_plus = in4[2]; b->omega_c_minus = in4[3];
+}
+
 void oref_init_equilibrium(oref_box* b)
 {
     /* gpu_init_equilibrium, LBM.cu:114-276 */

@@ -139,6 +139,12 @@ void oref_node_terms(const oref_params* p, double rho, double ux, double uy, dou
  * omega_c_plus, omega_c_minus. */
 void oref_rates(const oref_box* b, double* out4);
 
+/* Override the four rates (same order).  Tests use it to plant a mistake in
+ * the oracle on the CPU -- e.g. swapping omega_c_plus and omega_c_minus is
+ * the same as swapping their uses in every relaxation line -- and to show
+ * that the moment checks fail on it. */
+void oref_set_rates(oref_box* b, const double* in4);
+
 /* Set OpenMP thread count used by the oracle (0 = leave default). */
 void oref_set_threads(int n);
 

@@ -88,6 +88,7 @@ def lib() -> C.CDLL:
         L.oref_set_threads.argtypes = [C.c_int]
         L.oref_node_terms.argtypes = [C.POINTER(OrefParams)] + [C.c_double] * 11 + [dp, dp, dp]
         L.oref_rates.argtypes = [vp, dp]
+        L.oref_set_rates.argtypes = [vp, dp]
         _lib = L
     return _lib
 
@@ -207,6 +208,11 @@ class Box:
         r = np.zeros(4)
         lib().oref_rates(self._b, r.ctypes.data_as(C.POINTER(C.c_double)))
         return r
+
+    def set_rates(self, r) -> None:
+        a = np.ascontiguousarray(r, dtype=np.float64)
+        assert a.size == 4
+        lib().oref_set_rates(self._b, a.ctypes.data_as(C.POINTER(C.c_double)))
 
     def fft3d(self, src: int, dst: int, sign: int):
         dp = C.POINTER(C.c_double)

@@ -36,7 +36,32 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert E.lib().ehd_abi_version() == 1
+    assert E.lib().ehd_abi_version() == 2
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes structures have the C structures' sizes and field offsets
+    (a C program compiled against include/ehd.h prints them)."""
+    from ehd_amd import fno
+    structs = {"ehd_params": E.Params, "ehd_derived": E.Derived, "ehd_diag": E.Diag, "ehd_steady": E.Steady,
+               "ehd_profile": E.Profile, "ehd_tuning": E.Tuning, "ehd_fno_config": fno.Config}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ehd.h"', '#include "ehd_fno.h"',
+             'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                  text=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == __import__("ctypes").sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, f"{cname}.{fname}"
 
 
 def test_params_default_match_reference_header():

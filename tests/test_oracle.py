@@ -296,6 +296,92 @@ def test_wall_velocity_override():
     assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
 
 
+def _trt_taus(p):
+    """Relaxation times the TRT scheme must have for the box's transport
+    coefficients (Ginzburg's TRT: the even rate sets the shear viscosity
+    nu = cs^2 (tau+ - 1/2) dt, the odd rate of the charge set sets its
+    diffusivity D = cs^2 (tau_c- - 1/2) dt, and the magic parameter fixes the
+    other rate, Lambda = (tau+ - 1/2)(tau- - 1/2); LBM.h:79-80)."""
+    tp = p.nu / (p.cs_square * p.dt) + 0.5
+    tcm = p.diffu / (p.cs_square * p.dt) + 0.5
+    return dict(fp=tp, fm=0.5 + p.V / (tp - 0.5), cm=tcm, cp=0.5 + p.VC / (tcm - 0.5))
+
+
+def _trt_moment_residuals(b):
+    """Collide the box once and measure how each moment relaxed, against the
+    closed forms of a TRT collision with the relaxation times of _trt_taus:
+      sum xi h:        d = -(m - q v)/tau_c-           (v = u + K E)
+      sum xi xi h:     d = -(m - q (cs^2 I + v v))/tau_c+
+      sum xi_x xi_y xi_z h:  d = -m/tau_c-          (zero in the equilibrium)
+      sum xi xi f:     d = -(m - rho (cs^2 I + u u))/tau+ + dt (1 - 1/(2 tau+)) (u F + F u)
+      sum xi_x xi_y xi_z f:  d = -m/tau-            (zero in equilibrium and force)
+    on interior nodes, u = (sum xi f + F dt/2)/rho, F = q E + exf x.  Returns
+    the largest residual of each, relative to the largest change it predicts."""
+    p = b.p
+    xi, _ = _xi(p)
+    tau = _trt_taus(p)
+    f_pre = np.concatenate([b[O.F0][None], b[O.F1]])[:, 1:-1]
+    h_pre = np.concatenate([b[O.H0][None], b[O.H1]])[:, 1:-1]
+    E = np.stack([b[O.EX], b[O.EY], b[O.EZ]])[:, 1:-1]
+    b.collide_save()
+    f_post = np.concatenate([b[O.F0][None], b[O.F2]])[:, 1:-1]
+    h_post = np.concatenate([b[O.H0][None], b[O.H2]])[:, 1:-1]
+    rho, q = f_pre.sum(0), h_pre.sum(0)
+    F = q[None] * E
+    F[0] += p.exf
+    u = (np.einsum("dk,dzyx->kzyx", xi, f_pre) + 0.5 * p.dt * F) / rho[None]
+    v = u + p.K * E
+    eye = np.eye(3)[:, :, None, None, None]
+    x3 = xi[:, 0] * xi[:, 1] * xi[:, 2]
+
+    def m1(a): return np.einsum("dk,dzyx->kzyx", xi, a)
+    def m2(a): return np.einsum("dk,dl,dzyx->klzyx", xi, xi, a)
+    def m3(a): return np.einsum("d,dzyx->zyx", x3, a)
+
+    def resid(delta, want):
+        return float(np.max(np.abs(delta - want)) / np.max(np.abs(want)))
+
+    uu = np.einsum("kzyx,lzyx->klzyx", u, u)
+    vv = np.einsum("kzyx,lzyx->klzyx", v, v)
+    uF = np.einsum("kzyx,lzyx->klzyx", u, F)
+    cs2 = p.cs_square
+    return {
+        "h1": resid(m1(h_post) - m1(h_pre), -(m1(h_pre) - q[None] * v) / tau["cm"]),
+        "h2": resid(m2(h_post) - m2(h_pre), -(m2(h_pre) - q * (cs2 * eye + vv)) / tau["cp"]),
+        "h3": resid(m3(h_post) - m3(h_pre), -m3(h_pre) / tau["cm"]),
+        "f2": resid(m2(f_post) - m2(f_pre), -(m2(f_pre) - rho * (cs2 * eye + uu)) / tau["fp"]
+                    + p.dt * (1 - 0.5 / tau["fp"]) * (uF + uF.transpose(1, 0, 2, 3, 4))),
+        "f3": resid(m3(f_post) - m3(f_pre), -m3(f_pre) / tau["fm"]),
+    }
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(lam=1.0 / 12.0, exf=3.0), dict(lam_c=0.25, Fe=400.0)])
+def test_trt_relaxation_rates_closed_form(kw):
+    """Each moment of the collision relaxes with the rate the transport
+    coefficients fix: h's flux and odd moments with tau_c- (diffusivity),
+    h's second moment with tau_c+ (magic parameter Lambda_c), f's stress with
+    tau+ (viscosity), f's odd third moment with tau- (Lambda)."""
+    b = _box(6, 5, 7, **kw)
+    _random_state(b, np.random.default_rng(17))
+    r = _trt_moment_residuals(b)
+    assert max(r.values()) < 1e-9, r
+
+
+@pytest.mark.parametrize("swap", ["charge", "flow"])
+def test_trt_moment_checks_catch_swapped_rates(swap):
+    """A planted mistake in the oracle (on the CPU): swapping the even and odd
+    rates of one population set -- the same as swapping their uses in every
+    relaxation line of the collision -- must fail the closed forms above."""
+    b = _box(6, 5, 7)
+    _random_state(b, np.random.default_rng(17))
+    r = b.rates()
+    r = r[[1, 0, 2, 3]] if swap == "flow" else r[[0, 1, 3, 2]]
+    b.set_rates(r)
+    res = _trt_moment_residuals(b)
+    keys = ("f2", "f3") if swap == "flow" else ("h1", "h2", "h3")
+    assert min(res[k] for k in keys) > 1e-3, res
+
+
 # --------------------------------------------------------------------------- Poisson
 
 def _numpy_poisson(q, p):
