@@ -12,8 +12,12 @@ Prints one JSON line.  roofline: the fused LBM kernel (the dominant kernel),
 algorithmic bytes 440 B per node (2 x 27 populations read and written in
 FP32, q written, phi read) over its average launch time, taken from HIP
 events recorded on the box's stream around the launch of every 8th step of
-the timed region (--profile-every).  cpu_baseline: the FP64 CPU oracle
-(OpenMP) on a bounded sample of the same workload.
+the timed region (--profile-every).  copy_ceiling_gbs: a 2 GiB float4
+device-to-device copy in the same process (ehd_copy_ceiling), plain and
+nontemporal, the box's practical HBM ceiling.  fp64: the same box in the
+reference's precision (FP64 populations, 880 B per node for the LBM kernel).
+cpu_baseline: the FP64 CPU oracle (OpenMP, one thread per physical core,
+bound) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -52,20 +56,44 @@ def parse():
     ap.add_argument("--no-profile", action="store_true", help="no per-launch events (value only)")
     ap.add_argument("--profile-every", type=int, default=8,
                     help="HIP events around the LBM and Poisson launches of every n-th step (each event record idles the stream for a few microseconds; every launch does the same work)")
+    ap.add_argument("--fp64-steps", type=int, default=40,
+                    help="steps of the same box in FP64 for the fp64 sub-object (0: skip)")
     ap.add_argument("--workload", choices=("box", "ensemble"), default="box",
                     help="box: one 256x256x128 box (the headline line); ensemble: the 48-box sweep in one batch")
     return ap.parse_args()
 
 
+TRAFFIC_FILE = os.path.join("profiles", "lbm_traffic.json")
+
+
 def traffic_from_profiles(workload: str):
-    """HBM bytes per LBM launch measured with rocprofv3 PMC counters (profiles/)."""
-    path = os.path.join(ROOT, "profiles", "lbm_traffic.json")
+    """HBM bytes per LBM launch of this workload as two rocprofv3 PMC runs of
+    the same command measured them (FETCH_SIZE, then WRITE_SIZE, each scaled
+    by what the counter reads for a copy of known size), committed in
+    profiles/lbm_traffic.json with the command that made them; None if the
+    file holds no entry for the workload."""
     try:
-        with open(path) as f:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
             data = json.load(f)
         return data.get(workload, {}).get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+def physical_cores(cpus):
+    """Distinct physical cores among the logical CPUs `cpus` (sysfs topology)."""
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            with open(base + "physical_package_id") as f:
+                pkg = f.read().strip()
+            with open(base + "core_id") as f:
+                core = f.read().strip()
+        except OSError:
+            return len(cpus)
+        cores.add((pkg, core))
+    return len(cores)
 
 
 def cpu_model():
@@ -80,9 +108,16 @@ def cpu_model():
 
 
 def cpu_baseline(args, workload):
+    # one OpenMP thread per physical core this process may use, bound to its
+    # core (the OpenMP runtime reads these when the oracle library loads)
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = physical_cores(cpus)
+    limit = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = args.cpu_threads or (min(cores, limit) if limit else cores)
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ["OMP_PLACES"] = "cores"
     from oracle import pyoracle as O
 
-    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     O.set_threads(threads)
     p = O.make_params(args.nx, args.ny, args.nz, T=args.T, M=10.0, C_=10.0, Fe=4000.0, lam=0.25)
     b = O.Box(p)
@@ -99,7 +134,8 @@ def cpu_baseline(args, workload):
         total += dt
         rates.append(S * n / dt / 1e6)
     return {"value": round(sorted(rates)[1], 4), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "cpu": cpu_model(),
+            "cpu": cpu_model(), "logical_cpus": len(cpus), "physical_cores": cores,
+            "binding": "OMP_PROC_BIND=close OMP_PLACES=cores",
             "sample": f"FP64 oracle, {workload}, median of 3 runs of {n} coupled steps after 1 warm-up step, "
                       f"{total:.1f} s in all"}
 
@@ -160,6 +196,41 @@ def run_ensemble(args):
     print(json.dumps(line), flush=True)
 
 
+def run_box(E, args, prec, steps, warmup):
+    """One box of the workload stepped warmup + steps times; the timed region
+    is the `steps` coupled steps (inputs resident in HBM), bracketed by stream
+    synchronisation.  Returns wall time, the sampled kernel times and the
+    diagnostics."""
+    nx, ny, nz = args.nx, args.ny, args.nz
+    params = E.make_params(nx, ny, nz, T=args.T, C_=10.0, M=10.0, Fe=4000.0, Pr=10.0, lam=0.25,
+                           precision=prec, init_mode=E.INIT_HYDROSTATIC, seed=args.seed, amp=1e-3)
+    with E.Box(params) as box:
+        box.init()
+        box.step(warmup)
+        box.sync()
+        if not args.no_profile:
+            box.profile_enable(True, every=args.profile_every)
+        box.profile_reset()
+        t0 = time.perf_counter()
+        box.step(steps)
+        box.sync()
+        elapsed = time.perf_counter() - t0
+        prof = box.profile_read()
+        d = box.diagnostics()
+    return elapsed, prof, d
+
+
+def roofline(E, args, prec, prof, workload):
+    lbm_ms = prof.lbm_ms / prof.lbm_launches
+    S = args.nx * args.ny * args.nz
+    achieved = LBM_BYTES_PER_NODE[prec] * S / (lbm_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(workload),
+            "traffic_source": TRAFFIC_FILE,
+            "kernel": "lbm_f32s_kernel" if prec == E.FP32S else "lbm_f64_kernel",
+            "bytes_per_node": LBM_BYTES_PER_NODE[prec], "avg_launch_ms": round(lbm_ms, 4)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -175,23 +246,7 @@ def main():
     nx, ny, nz = args.nx, args.ny, args.nz
     S = nx * ny * nz
     workload = f"coupled EHD step {nx}x{ny}x{nz} {args.precision}"
-    params = E.make_params(nx, ny, nz, T=args.T, C_=10.0, M=10.0, Fe=4000.0, Pr=10.0, lam=0.25,
-                           precision=prec, init_mode=E.INIT_HYDROSTATIC, seed=args.seed, amp=1e-3)
-    box = E.Box(params)
-    box.init()
-    box.step(args.warmup)
-    box.sync()
-    if not args.no_profile:
-        box.profile_enable(True, every=args.profile_every)
-    box.profile_reset()
-    t0 = time.perf_counter()
-    box.step(args.steps)
-    box.sync()
-    elapsed = time.perf_counter() - t0
-    prof = box.profile_read()
-    d = box.diagnostics()
-    box.close()
-
+    elapsed, prof, d = run_box(E, args, prec, args.steps, args.warmup)
     mlups = S * args.steps / elapsed / 1e6
     ms_step = elapsed * 1e3 / args.steps
     line = {
@@ -204,16 +259,27 @@ def main():
         "hbm_frac_step": round(STEP_BYTES_PER_NODE[prec] * S * args.steps / elapsed / 1e9 / HBM_PEAK_GBS, 4),
     }
     if not args.no_profile and prof.lbm_launches > 0:
-        lbm_ms = prof.lbm_ms / prof.lbm_launches
-        achieved = LBM_BYTES_PER_NODE[prec] * S / (lbm_ms * 1e-3) / 1e9
-        line["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(achieved / HBM_PEAK_GBS, 4),
-                            "traffic": traffic_from_profiles(workload),
-                            "kernel": "lbm_f32s_kernel" if prec == E.FP32S else "lbm_f64_kernel",
-                            "avg_launch_ms": round(lbm_ms, 4)}
-        line["kernel_ms"] = {"lbm": round(lbm_ms, 4),
+        line["roofline"] = roofline(E, args, prec, prof, workload)
+        line["kernel_ms"] = {"lbm": line["roofline"]["avg_launch_ms"],
                              "poisson": round(prof.poisson_ms / max(prof.poisson_calls, 1), 4)}
     line["diag"] = {"current": d.current, "umax": d.umax}
+    # the box's own practical ceiling: a 2 GiB float4 copy, plain and nontemporal, same process
+    plain, nt = E.copy_ceiling(1 << 31, 20)
+    line["copy_ceiling_gbs"] = {"plain": round(plain, 1), "nontemporal": round(nt, 1)}
+    if "roofline" in line:
+        line["roofline"]["frac_of_copy_ceiling"] = round(line["roofline"]["achieved"] / max(plain, nt), 4)
+    if prec == E.FP32S and args.fp64_steps > 0:
+        # the reference's own precision on the same workload
+        e64, p64, _ = run_box(E, args, E.FP64, args.fp64_steps, 5)
+        sub = {"ms_per_step": round(e64 * 1e3 / args.fp64_steps, 4),
+               "MLUPS": round(S * args.fp64_steps / e64 / 1e6, 2), "steps": args.fp64_steps,
+               "hbm_frac_step": round(STEP_BYTES_PER_NODE[E.FP64] * S * args.fp64_steps / e64 / 1e9
+                                      / HBM_PEAK_GBS, 4)}
+        if not args.no_profile and p64.lbm_launches > 0:
+            r = roofline(E, args, E.FP64, p64, f"coupled EHD step {nx}x{ny}x{nz} fp64")
+            sub["roofline"] = {k: r[k] for k in ("kernel", "achieved", "frac", "avg_launch_ms", "traffic",
+                                                 "bytes_per_node")}
+        line["fp64"] = sub
     if args.cpu_steps > 0:
         line["cpu_baseline"] = cpu_baseline(args, workload)
     print(json.dumps(line), flush=True)

@@ -201,7 +201,7 @@ int main(int argc, char** argv)
         while (j < o.nsteps - 1 && !fires(j)) ++j;
         const long long n = j - i + 1;
         CALL(ehd_step(b, (int)n));
-        t += d.dt * (double)n;
+        for (long long k = 0; k < n; ++k) t = t + d.dt;   // one addition per step, as main.cu:208
         i = j;
         const bool save = !o.quiet_files && i % o.nsave == 1, dmd = i % o.ndmd == 1;
         const bool cur = i % o.print_current == 1;

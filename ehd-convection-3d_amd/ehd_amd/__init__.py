@@ -336,9 +336,14 @@ class Box:
         _check(lib().ehd_hydrostatic_reference(self._h, int(check_every), float(tol), int(max_steps), C.byref(v)))
         return v.value
 
-    def run_to_steady(self, check_every: int = 1000, tol: float = 1e-6, max_steps: int = 10**6) -> Steady:
+    def run_to_steady(self, check_every: int = 1000, tol: float = 1e-6, max_steps: int = 10**6,
+                      allow_nonfinite: bool = False) -> Steady:
+        """Steps to |dNu| < tol per chunk; a box the watchdog flags raises
+        EHD_ENONFINITE unless allow_nonfinite (then s.failed = 1)."""
         s = Steady()
-        _check(lib().ehd_run_to_steady(self._h, int(check_every), float(tol), int(max_steps), C.byref(s)))
+        rc = lib().ehd_run_to_steady(self._h, int(check_every), float(tol), int(max_steps), C.byref(s))
+        if not (allow_nonfinite and rc == -5):
+            _check(rc)
         return s
 
     def save_state(self, path: str):

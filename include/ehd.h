@@ -235,6 +235,7 @@ const char* ehd_last_error(void);
  * `bytes` for the duration of the call. */
 int ehd_copy_ceiling(size_t bytes, int reps, double* out2);
 
+
 /* ---- batches: many independent boxes stepped together ------------------
  * Stability maps and hysteresis sweeps run many small boxes of uneven size
  * and parameters (T, C, M, lateral wavelength via nx, ny).  A batch steps

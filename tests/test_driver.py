@@ -1,0 +1,210 @@
+"""The driver surface of main.cu, run as a program on the GPU.
+
+`driver/ehd_main` (main.cu's command line, time loop, output cadence
+`i % N == 1` and files; main.cu:166-239, LBM.cu:1427-1713) runs a 16x12x10
+FP64 box from the reference initial state (LBM.cu:69-96) with the reference's
+physics (T ~ 996, M = C = 10, Fe = 4000, Lambda = 1/12, Lambda_c = 1e-6).  The
+same run in the FP64 oracle, written at the same cadence with the reference's
+own fprintf format strings (parsed from LBM.cu/main.cu text into
+tests/golden/writers_golden.json), must give the same data.dat, data_end.dat,
+umax.dat, DMD files and "Current =" lines.  Text is compared token by token:
+identical, or one unit apart in the last printed digit (GPU and oracle agree to
+~1e-15 in FP64, which can move a printed digit only at a rounding boundary).
+"""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import ehd_amd as E
+from oracle import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "ehd-convection-3d_amd", "driver", "ehd_main")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "writers_golden.json")
+FIELD_KEYS = (O.RHO, O.Q, O.PHI, O.UX, O.UY, O.UZ, O.EX, O.EY, O.EZ)
+
+
+def _extrap(v, bottom=True, top=True):
+    """Wall rows rebuilt linearly from the two nearest interior rows before
+    writing (LBM.cu:1457-1472, 1582-1585, 1690-1694)."""
+    v = v.copy()
+    if bottom:
+        v[0] = 2.0 * v[1] - v[2]
+    if top:
+        v[-1] = 2.0 * v[-2] - v[-3]
+    return v
+
+
+class Writers:
+    """The reference's writers restated with its own format strings."""
+
+    def __init__(self, fmt, shape, spacing):
+        self.f = fmt
+        self.nz, self.ny, self.nx = shape
+        self.dx, self.dy, self.dz = spacing
+
+    def _c(self, f9):
+        RHO, Q, PHI, UX, UY, UZ, EX, EY, EZ = range(9)
+        c = [f9[k].copy() for k in range(9)]
+        for k in (RHO, Q, UX, UY, UZ):
+            c[k] = _extrap(c[k])
+        return c
+
+    def tecplot(self, t, f9, first):
+        tec = self.f["tecplot"]
+        out = (tec[0] % self.f["variables"]) if first else ""
+        out += tec[1] + tec[2] % (t, self.nx, self.ny, self.nz)
+        RHO, Q, PHI, UX, UY, UZ, EX, EY, EZ = self._c(f9)
+        for z in range(self.nz):
+            for y in range(self.ny):
+                for x in range(self.nx):
+                    out += tec[3] % (self.dx * x, self.dy * y, self.dz * z, UX[z, y, x], UY[z, y, x], UZ[z, y, x],
+                                     RHO[z, y, x], Q[z, y, x], PHI[z, y, x], EX[z, y, x], EY[z, y, x], EZ[z, y, x])
+        return out
+
+    def data_end(self, t, f9):
+        RHO, Q, PHI, UX, UY, UZ, EX, EY, EZ = (a.reshape(-1) for a in self._c(f9))
+        return "".join(self.f["data_end"] % (t, UX[i], UY[i], UZ[i], RHO[i], Q[i], PHI[i], EX[i], EY[i], EZ[i])
+                       for i in range(UX.size))
+
+    def dmd(self, t, uz):
+        w = _extrap(uz).reshape(-1)
+        return "".join(self.f["dmd"] % (t, v) for v in w)
+
+    def umax(self, t, uz):
+        m = 0.0
+        for v in _extrap(uz, bottom=False).reshape(-1):
+            m = m if m > v else v
+        return self.f["umax"] % (t, m)
+
+
+def _unit(tok):
+    """One unit in the last printed digit of a %g / %f token."""
+    m = re.fullmatch(r"[-+]?(\d*)\.?(\d*)(?:e([-+]\d+))?", tok)
+    dec = len(m.group(2)) if m else 0
+    ex = int(m.group(3)) if m and m.group(3) else 0
+    return 10.0 ** (ex - dec)
+
+
+def _column_scales(lines):
+    """Largest |value| of each numeric column (by token position) of a text."""
+    scale = {}
+    for line in lines:
+        for k, tok in enumerate(line.split()):
+            try:
+                scale[k] = max(scale.get(k, 0.0), abs(float(tok)))
+            except ValueError:
+                pass
+    return scale
+
+
+def compare_text(got, want, what, rel=1e-12):
+    """Token by token: identical, or one unit apart in the last printed digit,
+    or within rel of the largest |value| of that column in the expected text
+    (FP64 values that are themselves differences of nearly equal numbers --
+    the wall extrapolations 2 v1 - v2 of a smooth profile -- carry the FP64
+    rounding of their operands, not of their own size).  Returns the number
+    of tokens that were not identical."""
+    if got == want:
+        return 0
+    gl, wl = got.splitlines(), want.splitlines()
+    assert len(gl) == len(wl), f"{what}: {len(gl)} lines, expected {len(wl)}"
+    scale = _column_scales(wl)
+    near = 0
+    for n, (a, b) in enumerate(zip(gl, wl)):
+        if a == b:
+            continue
+        ta, tb = a.split(), b.split()
+        assert len(ta) == len(tb), f"{what} line {n + 1}: {a!r} vs {b!r}"
+        for k, (x, y) in enumerate(zip(ta, tb)):
+            if x == y:
+                continue
+            try:
+                fx, fy = float(x), float(y)
+            except ValueError:
+                raise AssertionError(f"{what} line {n + 1}: {a!r} vs {b!r}")
+            tol = max(1.01 * max(_unit(x), _unit(y)), rel * scale.get(k, 0.0))
+            assert abs(fx - fy) <= tol, f"{what} line {n + 1} column {k + 1}: {x} vs {y} (tolerance {tol:.3g})"
+            near += 1
+    return near
+
+
+def _oracle_fields(ob):
+    return np.stack([ob[k].copy() for k in FIELD_KEYS])
+
+
+def test_restated_writers_reproduce_golden_text():
+    """The writers above give the golden text of writers_golden.json on its
+    own synthetic fields (CPU)."""
+    g = json.load(open(GOLDEN))
+    nx, ny, nz = g["grid"]
+    f9 = np.array([[float.fromhex(v) for v in row] for row in g["fields9_hex"]]).reshape(9, nz, ny, nx)
+    t = float.fromhex(g["t"])
+    W = Writers(g["formats"], (nz, ny, nx), g["spacing"])
+    assert W.tecplot(t, f9, True) == g["tecplot_first"]
+    assert W.tecplot(t, f9, False) == g["tecplot_next"]
+    assert W.data_end(t, f9) == g["data_end"]
+    assert W.dmd(t, f9[5]) == g["dmd"]
+    assert W.umax(t, f9[5]) == g["umax"]
+    assert compare_text("1.23456e-05 0.123456", "1.23457e-05 0.123455", "near") == 2
+    with pytest.raises(AssertionError):
+        compare_text("1.23456e-05", "1.23458e-05", "far")
+    # a cancellation-noise value next to a column of size 1e-2: within 1e-12 of the column
+    assert compare_text("1 7.1063e-15\n2 0.01", "1 7.13174e-15\n2 0.01", "noise") == 1
+    with pytest.raises(AssertionError):
+        compare_text("1 7.1063e-12\n2 0.01", "1 7.13174e-12\n2 0.01", "noise")
+
+
+@pytest.mark.gpu
+def test_driver_reference_run_matches_oracle(tmp_path):
+    nx, ny, nz, nsteps, nsave, ndmd, pcur = 16, 12, 10, 7, 3, 4, 2
+    golden = json.load(open(GOLDEN))
+    dp = E.default_params()      # the reference's physics (LBM.h:31-86)
+    run = subprocess.run([DRIVER, "--nx", str(nx), "--ny", str(ny), "--nz", str(nz), "--steps", str(nsteps),
+                          "--nsave", str(nsave), "--ndmd", str(ndmd), "--print-current", str(pcur),
+                          "--init", "reference", "--precision", "fp64", "--outdir", str(tmp_path)],
+                         capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stderr
+    print(run.stdout)
+
+    op = O.make_params(nx, ny, nz, T=dp.T, M=dp.M, C_=dp.C, Fe=dp.Fe, lam=dp.lambda_f, lam_c=dp.lambda_c)
+    ob = O.Box(op)
+    ob.initialization()
+    ob.init_equilibrium()
+    W = Writers(golden["formats"], (nz, ny, nx), (op.dx, op.dy, op.dz))
+    t = 0.0
+    data = W.tecplot(t, _oracle_fields(ob), True)
+    umax, currents, dmd, iters = "", [], {}, []
+    for i in range(nsteps):            # main.cu:181-239
+        ob.step(1)
+        t = t + op.dt
+        f9 = _oracle_fields(ob)
+        if i % nsave == 1:
+            data += W.tecplot(t, f9, False)
+        if i % ndmd == 1:
+            iters.append((i, "%g" % t))
+            dmd["%g.dat" % t] = W.dmd(t, f9[5])
+        if i % pcur == 1:
+            currents.append("%g" % ob.current())
+            umax += W.umax(t, f9[5])
+    f9 = _oracle_fields(ob)
+    data += W.tecplot(t, f9, False)
+    umax += W.umax(t, f9[5])
+    end = W.data_end(t, f9)
+
+    near = 0
+    near += compare_text(open(tmp_path / "data.dat").read(), data, "data.dat")
+    near += compare_text(open(tmp_path / "umax.dat").read(), umax, "umax.dat")
+    near += compare_text(open(tmp_path / "data_end.dat").read(), end, "data_end.dat")
+    for name, text in dmd.items():
+        near += compare_text(open(tmp_path / name).read(), text, name)
+    got_cur = re.findall(r"^Current = (\S+)$", run.stdout, flags=re.M)
+    near += compare_text("\n".join(got_cur), "\n".join(currents), "Current lines")
+    got_it = re.findall(r"^Iteration: (\d+), physical time: (\S+)\.$", run.stdout, flags=re.M)
+    assert [(int(a), b) for a, b in got_it] == iters
+    print(f"driver vs oracle: {len(dmd)} DMD files, {len(currents)} current lines; "
+          f"{near} tokens one unit apart in the last digit, the rest identical")

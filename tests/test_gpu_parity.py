@@ -138,6 +138,16 @@ def test_checkpoint_restart_bitexact(tmp_path, precision):
     with E.Box(E.make_params(16, 12, 12, precision=precision)) as c:
         with pytest.raises(E.EhdError):
             c.load_state(path)
+    # same grid and precision, other physics: the populations (FP32S: stored
+    # against the box's hydrostatic tables) would be wrong, so the load fails
+    p = ob.p
+    for other in (dict(T=201.0), dict(C_=11.0), dict(lam=0.2), dict(exf=1.0), dict(cfl=0.02)):
+        kw = dict(T=200.0, C_=10.0, M=10.0, Fe=4000.0, lam=p.V, lam_c=p.VC, uw=p.uw, exf=p.exf)
+        kw.update(other)
+        with E.Box(E.make_params(16, 12, 10, precision=precision, init_mode=E.INIT_FIELDS, **kw)) as c:
+            with pytest.raises(E.EhdError) as ei:
+                c.load_state(path)
+            assert ei.value.code == -1 and "differs" in str(ei.value), str(ei.value)
 
 
 @pytest.mark.parametrize("case", [
@@ -214,7 +224,9 @@ def test_random_population_state_bitexact():
 
 
 @pytest.mark.parametrize("shape", [(8, 6, 7), (48, 48, 32), (80, 80, 12), (64, 64, 32), (122, 122, 10), (96, 96, 8),
-                                   (64, 48, 200), (128, 128, 65), (16, 8, 300)])
+                                   (64, 48, 200), (128, 128, 65), (16, 8, 300),
+                                   # the compile-time plans of the bench box and of the largest configuration
+                                   (256, 256, 34), (256, 256, 128), (512, 512, 12)])
 @pytest.mark.parametrize("precision", [E.FP64, E.FP32S])
 def test_poisson_vs_oracle(shape, precision):
     nx, ny, nz = shape
@@ -314,30 +326,66 @@ def _coupled_errors(ga, ob):
             "phi": rel_l2(get(ga, O.PHI, "phi"), ob[O.PHI])}
 
 
-@pytest.mark.parametrize("nsteps", [1000])
-def test_coupled_fp64_config1_vs_oracle(nsteps):
+def test_coupled_fp64_config1_vs_oracle():
     """64x64x32, T=200, C=10, M=10, Fe=4000, Lambda=1/4, hydrostatic charge and potential,
-    uniform rho0, 1e-3 velocity noise; 1000 coupled FP64 steps on the GPU and in the oracle.
+    uniform rho0, 1e-3 velocity noise; 1000 coupled FP64 steps on the GPU and in the oracle,
+    compared every 100 steps.
 
     Bar: phi and q within 1e-12 rel-L2.  For u the bar is 1e-12 or the FP64 floor of this
     problem, whichever is larger: the floor is the distance between two FP64 evaluations
-    of the same discrete step inside the oracle (odd-extension FFT Poisson vs x,y FFT plus
-    tridiagonal z solve), measured here on the same start.  The GPU must not be farther
-    from the oracle than twice that floor."""
+    of the same discrete step inside the oracle (odd-extension FFT Poisson, step, vs x,y FFT
+    plus tridiagonal z solve, step_tridiag), on the same start.  The GPU must not be farther
+    than twice that floor from either: from the FFT leg, which restates the reference, and
+    from the tridiagonal leg, whose algorithm the GPU shares."""
     ob = oracle_box(64, 64, 32)
     ob.init_equilibrium()
-    with gpu_box(ob) as b:
-        b.step(nsteps)
-        g = b.read_fields(("q", "phi", "ux", "uy", "uz"))
     ot = oracle_box(64, 64, 32)
     ot.init_equilibrium()
+    with gpu_box(ob) as b:
+        for k in range(1, 11):
+            b.step(100)
+            ob.step(100)
+            ot.step_tridiag(100)
+            g = b.read_fields(("q", "phi", "ux", "uy", "uz"))
+            errs, tri, floor = _coupled_errors(g, ob), _coupled_errors(g, ot), _coupled_errors(ot, ob)
+            print(f"step {100 * k}: GPU vs FFT oracle {errs}; GPU vs tridiagonal oracle {tri}; "
+                  f"oracle floor {floor}", flush=True)
+    for e in (errs, tri):
+        assert e["phi"] < 1e-12 and e["q"] < 1e-12, (e, floor)
+        assert e["u"] < max(1e-12, 2.0 * floor["u"]), (e, floor)
+
+
+@pytest.mark.timeout(400)
+def test_coupled_fp64_reference_configuration():
+    """The reference's own configuration (LBM.h:31-86): 122x122x101, T ~ 996,
+    M = C = 10, Fe = 4000, Lambda = 1/12, Lambda_c = 1e-6, from the reference
+    initial state (rho0, q = 0, linear phi, u = 0, E = 0; LBM.cu:69-96);
+    300 coupled FP64 steps on the GPU and in the oracle (the charge front
+    leaves the injection wall).  Bar: q, phi within 1e-12 rel-L2, u within
+    1e-12 or twice the oracle's own FP64 floor (FFT vs tridiagonal Poisson)."""
+    dp = E.default_params()
+    nx, ny, nz, nsteps = dp.nx, dp.ny, dp.nz, 300
+    op = O.make_params(nx, ny, nz, T=dp.T, M=dp.M, C_=dp.C, Fe=dp.Fe, lam=dp.lambda_f, lam_c=dp.lambda_c)
+    ob = O.Box(op)
+    ob.initialization()
+    ob.init_equilibrium()
+    with E.Box(dp) as b:
+        b.init()
+        compare_pops(b, ob, "reference configuration init")
+        b.step(nsteps)
+        g = b.read_fields(("q", "phi", "ux", "uy", "uz"))
     ob.step(nsteps)
+    ot = O.Box(op)
+    ot.initialization()
+    ot.init_equilibrium()
     ot.step_tridiag(nsteps)
-    errs = _coupled_errors(g, ob)
-    floor = _coupled_errors(ot, ob)
-    print(f"coupled FP64 {nsteps} steps rel-L2 GPU vs oracle: {errs}; oracle FP64 floor: {floor}")
-    assert errs["phi"] < 1e-12 and errs["q"] < 1e-12, errs
-    assert errs["u"] < max(1e-12, 2.0 * floor["u"]), (errs, floor)
+    errs, tri, floor = _coupled_errors(g, ob), _coupled_errors(g, ot), _coupled_errors(ot, ob)
+    print(f"122x122x101 reference configuration, {nsteps} steps: GPU vs FFT oracle {errs}; "
+          f"GPU vs tridiagonal oracle {tri}; oracle floor {floor}; q max {np.abs(ob[O.Q]).max():.4g}")
+    assert np.abs(ob[O.Q][1:-1]).max() > 0
+    for e in (errs, tri):
+        assert e["phi"] < 1e-12 and e["q"] < 1e-12, (e, floor)
+        assert e["u"] < max(1e-12, 2.0 * floor["u"]), (e, floor)
 
 
 @pytest.mark.parametrize("hd", [0, 1])
