@@ -90,6 +90,7 @@ EXPORTS = (
     "ehd_init_consistent", "ehd_batch_init_consistent",
     "ehd_batch_create", "ehd_batch_destroy", "ehd_batch_init", "ehd_batch_step", "ehd_batch_diagnostics",
     "ehd_batch_run_to_steady", "ehd_batch_box", "ehd_batch_tuning", "ehd_copy_ceiling",
+    "ehd_poisson_noalloc",
     # host-only file layer, include/ehd_io.h
     "ehd_io_tecplot", "ehd_io_data_end", "ehd_io_read_end", "ehd_io_dmd", "ehd_io_umax", "ehd_io_current",
 )
@@ -156,6 +157,7 @@ def lib() -> C.CDLL:
     L.ehd_last_error.restype = C.c_char_p
     L.ehd_abi_version.argtypes = []
     L.ehd_copy_ceiling.argtypes = [C.c_size_t, C.c_int, dp]
+    L.ehd_poisson_noalloc.argtypes = [C.POINTER(Params), C.c_void_p, C.c_void_p, C.c_void_p]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("ehd_destroy", "ehd_last_error", "ehd_io_current", "ehd_batch_destroy", "ehd_batch_box"):

@@ -235,6 +235,22 @@ const char* ehd_last_error(void);
  * `bytes` for the duration of the call. */
 int ehd_copy_ceiling(size_t bytes, int reps, double* out2);
 
+/* ---- the Poisson solve without allocation ----------------------------------
+ * No counterpart in the reference: a variant of ehd_poisson for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It solves the FP64 Poisson
+ * problem of ehd_poisson -- the same kernels, twiddles and job -- for the
+ * grid and physics of *p (precision must be EHD_FP64) on caller-owned device
+ * buffers: q (S doubles, [z][y][x], the charge) in, phi (S doubles, walls
+ * included) out, on `stream` (a hipStream_t; NULL for the null stream).  Its
+ * workspace (job table, twiddles, spectrum) is one file-scope device array of
+ * EHD_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a grid
+ * whose workspace (16 (nx/2+1) ny (nz-2) bytes of spectrum plus the twiddle
+ * tables) does not fit is EHD_EINVAL (256x256x128 fits).  It calls no
+ * allocating routine.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU race.  Returns after the stream has finished the solve. */
+#define EHD_NOALLOC_SCRATCH_BYTES ((size_t)(64u << 20) + (size_t)(64u << 10))
+int ehd_poisson_noalloc(const ehd_params* p, const double* q, double* phi, void* stream);
 
 /* ---- batches: many independent boxes stepped together ------------------
  * Stability maps and hysteresis sweeps run many small boxes of uneven size
