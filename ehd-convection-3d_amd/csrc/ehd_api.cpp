@@ -519,6 +519,18 @@ int ehd_create(const ehd_params* p, ehd_box** out)
             rs[z] = p->rho0 + J * (b->der.dz * z - 0.5 * p->lz) / b->der.cs_square;
         }
         qs[nz - 1] = qs[nz - 2];   // the top plane holds copies of plane nz-2
+        // The shifts only have to be near the hydrostatic state, so take them
+        // representable in float: the kernel's rho = rs + sum g and
+        // q = qs + sum k then add an exact float shift (a rounded one is off
+        // by up to half an ulp of ~rho0 on a whole plane, the same on every
+        // node and every step -- a systematic density and force error that
+        // drives the plane-average velocity), and the differences between
+        // neighbouring planes, which re-shift pulled populations, are exact
+        // in float as well (Sterbenz).
+        for (int z = 0; z < nz; ++z) {
+            rs[z] = (double)(float)rs[z];
+            qs[z] = (double)(float)qs[z];
+        }
         std::vector<float> tf(6 * (size_t)nz);
         for (int z = 0; z < nz; ++z) {
             const int zm = (z == 0) ? nz - 1 : z - 1, zp = (z == nz - 1) ? 0 : z + 1;

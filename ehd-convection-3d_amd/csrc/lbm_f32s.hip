@@ -33,8 +33,21 @@
 namespace ehd {
 namespace {
 
+// The float constants keep the lattice's exact identities.  The D3Q27
+// weights 8/27, 2/27, 1/54, 1/216 share one mantissa, so in float all four
+// are too large by the same 7.5e-9: sum w = 1 + 7.5e-9 and, with A rounded
+// on its own, A sum_d w_d c_dz^2 = CFL (1 - 1.5e-8).  An equilibrium built on
+// them conserves neither the density deviation nor the momentum: every
+// collision scales the density deviation by 1 + om+ 7.5e-9 and the momentum
+// by 1 - om- 1.5e-8, which on a plane-uniform wave (every node of a plane
+// rounds alike, nothing averages out) grows into a relative velocity error of
+// ~4e-6 per 1000 steps against the FP64 path.  So w0 = 1 - 6 ws - 12 wa - 8 wd
+// of the float ws, wa, wd, and A = CFL / sum w c_z^2 of the same floats, are
+// carried as hi + lo float pairs (one extra FMA where they are used): mass
+// and momentum of the equilibrium then hold to double rounding.
 struct F32Consts {
-    float dt, cflinv, A;      // A = 1/(CFL cs^2)
+    float dt, cflinv, A, Alo; // A = 1/(CFL cs^2) = CFL / sum_d w_d c_dz^2 (hi + lo)
+    float w0lo;               // w[0] + w0lo = 1 - 6 w[1] - 12 w[2] - 8 w[3]
     float inv2cs2;            // 1/(2 cs^2)
     float B;                  // 1/(CFL^2 cs^4)
     float ics2;               // 1/cs^2
@@ -262,9 +275,10 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
 
     // rest populations
     {
-        const float geq0 = c.w[0] * (drho - rho * U2);
-        const float keq0 = c.w[0] * (dqe - qe * Uc2);
-        const float F0 = -c.w[0] * uF;
+        const float xg = drho - rho * U2, xk = dqe - qe * Uc2;
+        const float geq0 = fmaf(c.w[0], xg, c.w0lo * xg);
+        const float keq0 = fmaf(c.w[0], xk, c.w0lo * xk);
+        const float F0 = -fmaf(c.w[0], uF, c.w0lo * uF);
         const float g0n = g[0] - c.omp * (g[0] - geq0) + c.dt * c.sp * F0;
         const float k0n = k[0] - c.ocp * (k[0] - keq0);
         f0B[s] = fwall ? g[0] : g0n;
@@ -280,13 +294,13 @@ __device__ __forceinline__ void lbm_f32s_node(const LbmParams& P, const F32Const
         const float cu = (float)CX[d] * ux + (float)CY[d] * uy + (float)CZ[d] * uz;
         const float cc = (float)CX[d] * ucx + (float)CY[d] * ucy + (float)CZ[d] * ucz;
         const float cF = (float)CX[d] * Fx + (float)CY[d] * Fy + (float)CZ[d] * Fz;
-        const float tt = c.A * cu, tc = c.A * cc;
+        const float tt = fmaf(c.A, cu, c.Alo * cu), tc = fmaf(c.A, cc, c.Alo * cc);
         const float geqp = w * (drho + rho * (0.5f * tt * tt - U2));
         const float geqm = w * rho * tt;
         const float keqp = w * (dqe + qe * (0.5f * tc * tc - Uc2));
         const float keqm = w * qe * tc;
         const float Fp = w * (c.B * cu * cF - uF);
-        const float Fm = w * (c.A * cF);
+        const float Fm = w * fmaf(c.A, cF, c.Alo * cF);
         const float gp = 0.5f * (g[d] + g[e]), gmn = 0.5f * (g[d] - g[e]);
         const float kp = 0.5f * (k[d] + k[e]), kmn = 0.5f * (k[d] - k[e]);
         const float rp = c.omp * (gp - geqp), rm = c.omm * (gmn - geqm);
@@ -376,7 +390,15 @@ F32Consts make_f32consts(const LbmParams& P)
     const double cs2 = P.cs_square, cfl = P.cfl;
     c.dt = (float)P.dt;
     c.cflinv = (float)(1.0 / cfl);
-    c.A = (float)(1.0 / (cfl * cs2));
+    {
+        // identities of the float weights (see F32Consts)
+        const double ws = (float)P.w[1], wa = (float)P.w[2], wd = (float)P.w[3];
+        const double w0 = 1.0 - 6.0 * ws - 12.0 * wa - 8.0 * wd;
+        const double A = cfl / (2.0 * ws + 8.0 * wa + 8.0 * wd);
+        c.A = (float)A;
+        c.Alo = (float)(A - (double)c.A);
+        c.w0lo = (float)(w0 - (double)(float)w0);
+    }
     c.inv2cs2 = (float)(0.5 / cs2);
     c.B = (float)(1.0 / (cfl * cfl * cs2 * cs2));
     c.ics2 = (float)(1.0 / cs2);
@@ -389,7 +411,9 @@ F32Consts make_f32consts(const LbmParams& P)
     c.sp = (float)(1.0 - 0.5 * P.dt * P.omega_plus);
     c.sm = (float)(1.0 - 0.5 * P.dt * P.omega_minus);
     for (int i = 0; i < 4; ++i) {
-        c.w[i] = (float)P.w[i];
+        c.w[i] = (i == 0) ? (float)(1.0 - 6.0 * (double)(float)P.w[1] - 12.0 * (double)(float)P.w[2] -
+                                    8.0 * (double)(float)P.w[3])
+                          : (float)P.w[i];
         c.multi[i] = (float)P.multi[i];
         c.injk[i] = (float)(2.0 * P.w[i] * (P.charge0 - P.qs0));
     }

@@ -452,51 +452,69 @@ def test_fp32s_vs_fp64_developed_convection():
     assert max(errs["u"], errs["q"], errs["phi"]) < 2e-4, errs
 
 
-@pytest.mark.timeout(300)   # ~45 s
-def test_fp32s_vs_fp64_256x256x128_20000_steps():
-    """The bench box's own start (256x256x128, T=400, C=10, hydrostatic q and
-    phi, 1e-3 velocity noise from seed 2) stepped 20,000 times in both
-    precisions from the same fields.  The velocity of this start oscillates
-    (umax swings between ~5e-2 and ~2e-5 over the run), so the relative error
-    of u is read at the end, step 20,000, where umax is ~3e-4.  Bar: u, q, phi
-    within 1e-5 rel-L2, Nu and current within 0.1 %."""
-    def params_of(prec, init=E.INIT_FIELDS):
-        return E.make_params(256, 256, 128, T=400.0, C_=10.0, lam=0.25, precision=prec, init_mode=init, seed=2,
-                             amp=1e-3)
+def _fp32s_vs_fp64_checkpoints(params_of, nsteps, every, what):
+    """Both precisions from the same start (the FP64 path's hydrostatic start
+    with its seeded noise), compared every `every` steps.  At every
+    checkpoint: q and phi within 1e-5 rel-L2, Nu and current within 0.1 %, and
+    u within 1e-5 of the run's velocity scale, ||u_FP32S - u_FP64|| /
+    max over checkpoints of ||u_FP64||.  Most of u in these starts is a
+    plane-uniform standing acoustic wave (the start is not the discrete
+    equilibrium under the force); where it passes through zero ||u|| drops
+    ~1000-fold, and the instantaneous rel-L2 of u -- for two schemes whose
+    waves differ in phase by d, about d tan(wt) -- has no bound there.  The
+    instantaneous u error is printed at every checkpoint as well."""
     with E.Box(params_of(E.FP64, E.INIT_HYDROSTATIC)) as b:
         b.init()
         start = _nine(b.read_fields())
+    rows = []
     with E.Box(params_of(E.FP64)) as r, E.Box(params_of(E.FP32S)) as a:
         r.init(start)
         a.init(start)
-        for k in range(5000, 20001, 5000):
-            r.step(5000)
-            a.step(5000)
-            print(f"step {k}", flush=True)
-        fr = r.read_fields(("q", "phi", "ux", "uy", "uz"))
-        fa = a.read_fields(("q", "phi", "ux", "uy", "uz"))
-        dr, da = r.diagnostics(), a.diagnostics()
-    errs = dict(_field_errors(fa, fr), Nu=abs(da.nusselt / dr.nusselt - 1), current=abs(da.current / dr.current - 1))
-    print(f"256x256x128 from noise, FP64 Nu {dr.nusselt:.6f} umax {dr.umax:.3e}; FP32S vs FP64 after 20000 steps: {errs}")
-    assert errs["u"] < 1e-5 and errs["q"] < 1e-5 and errs["phi"] < 1e-5, errs
-    assert errs["Nu"] < 1e-3 and errs["current"] < 1e-3, errs
+        for k in range(every, nsteps + 1, every):
+            r.step(every)
+            a.step(every)
+            fr = r.read_fields(("q", "phi", "ux", "uy", "uz"))
+            fa = a.read_fields(("q", "phi", "ux", "uy", "uz"))
+            dr, da = r.diagnostics(), a.diagnostics()
+            ur = np.stack([fr["ux"], fr["uy"], fr["uz"]])[:, 1:-1]
+            ua = np.stack([fa["ux"], fa["uy"], fa["uz"]])[:, 1:-1]
+            errs = dict(_field_errors(fa, fr), Nu=abs(da.nusselt / dr.nusselt - 1),
+                        current=abs(da.current / dr.current - 1))
+            rows.append((k, errs, float(np.linalg.norm(ua - ur)), float(np.linalg.norm(ur)), dr.umax))
+    uscale = max(r[3] for r in rows)
+    for k, errs, du, nu, umax in rows:
+        errs["u_scale"] = du / uscale
+        print(f"{what} step {k}: umax {umax:.3e} |u|/max|u| {nu / uscale:.2e} "
+              + " ".join(f"{n} {v:.2e}" for n, v in errs.items()), flush=True)
+    for k, errs, du, nu, umax in rows:
+        assert errs["u_scale"] < 1e-5 and errs["q"] < 1e-5 and errs["phi"] < 1e-5, (k, errs)
+        assert errs["Nu"] < 1e-3 and errs["current"] < 1e-3, (k, errs)
+    return rows
 
 
+@pytest.mark.timeout(400)   # ~60 s
+def test_fp32s_vs_fp64_256x256x128_20000_steps():
+    """Config 2: the bench box's own start (256x256x128, T=400, C=10,
+    hydrostatic q and phi, uniform rho0, 1e-3 velocity noise from seed 2)
+    stepped 20,000 times in both precisions, the bar held at every 1,000-step
+    checkpoint.  The start is not the discrete equilibrium under the force,
+    so a plane-uniform acoustic wave carries most of u (umax swings between
+    ~5e-2 and ~1e-4)."""
+    def params_of(prec, init=E.INIT_FIELDS):
+        return E.make_params(256, 256, 128, T=400.0, C_=10.0, lam=0.25, precision=prec, init_mode=init, seed=2,
+                             amp=1e-3)
+    _fp32s_vs_fp64_checkpoints(params_of, 20000, 1000, "256x256x128")
+
+
+@pytest.mark.timeout(600)   # ~150 s
 def test_fp32s_vs_fp64_512x512x128():
-    """The largest configuration (512x512x128, 33.5 M nodes; T=1000, C=10,
-    hydrostatic start + 1e-3 noise from seed 5): 1000 steps in both
-    precisions from the same fields.  Bar: u, q, phi within 1e-5 rel-L2, Nu
-    and current within 0.1 %."""
+    """Config 5: the largest configuration (512x512x128, 33.5 M nodes; T=1000,
+    C=10, hydrostatic start + 1e-3 noise from seed 5) over its full 10,000
+    steps in both precisions, the bar held at every 1,000-step checkpoint."""
     def params_of(prec, init=E.INIT_FIELDS):
         return E.make_params(512, 512, 128, T=1000.0, C_=10.0, lam=0.25, precision=prec, init_mode=init, seed=5,
                              amp=1e-3)
-    with E.Box(params_of(E.FP64, E.INIT_HYDROSTATIC)) as b:
-        b.init()
-        start = _nine(b.read_fields())
-    errs, dr, _, _ = _fp32s_vs_fp64(params_of, start, 1000)
-    print(f"512x512x128, FP64 Nu {dr.nusselt:.6f}; FP32S vs FP64 after 1000 steps: {errs}")
-    assert errs["u"] < 1e-5 and errs["q"] < 1e-5 and errs["phi"] < 1e-5, errs
-    assert errs["Nu"] < 1e-3 and errs["current"] < 1e-3, errs
+    _fp32s_vs_fp64_checkpoints(params_of, 10000, 1000, "512x512x128")
 
 
 def test_fp32s_vs_fp64_256x256x128():
