@@ -226,7 +226,7 @@ def test_random_population_state_bitexact():
 @pytest.mark.parametrize("shape", [(8, 6, 7), (48, 48, 32), (80, 80, 12), (64, 64, 32), (122, 122, 10), (96, 96, 8),
                                    (64, 48, 200), (128, 128, 65), (16, 8, 300),
                                    # the compile-time plans of the bench box and of the largest configuration
-                                   (256, 256, 34), (256, 256, 128), (512, 512, 12)])
+                                   (256, 256, 34), (256, 256, 128), (512, 512, 12), (512, 512, 40)])
 @pytest.mark.parametrize("precision", [E.FP64, E.FP32S])
 def test_poisson_vs_oracle(shape, precision):
     nx, ny, nz = shape
