@@ -5,7 +5,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -16,19 +15,6 @@
 #include "fft_plan.h"
 
 using namespace ehd;
-
-namespace {
-thread_local char g_err[512] = "";
-}  // namespace
-
-int ehd::set_error(int code, const char* fmt, ...)
-{
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_err, sizeof(g_err), fmt, ap);
-    va_end(ap);
-    return code;
-}
 
 namespace {
 
@@ -401,7 +387,6 @@ int ckpt_io(ehd_box* b, FILE* f, bool save)
 extern "C" {
 
 int ehd_abi_version(void) { return EHD_ABI_VERSION; }
-const char* ehd_last_error(void) { return g_err; }
 
 int ehd_params_default(ehd_params* p)
 {
