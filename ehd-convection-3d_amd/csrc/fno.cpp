@@ -66,8 +66,11 @@ struct ehd_fno {
     float *dv0 = nullptr, *dv1 = nullptr, *dh = nullptr;
     double* part = nullptr;
     float* wscr = nullptr;        // split-K partial sums of the weight gradients
+    float* red = nullptr;         // partial sums of sum_slices and channel_sum
     long long wscr_n = 0;
     int ksplit = 0, nsplit = 1;   // K chunk of the weight-gradient GEMMs over the grid points
+    ehd_fno_work work{};          // GEMM work since the last reset
+    int resident = 0;             // samples of the last inputs/targets copied to the device
 };
 
 namespace {
@@ -110,6 +113,11 @@ hipError_t rgemm(ehd_fno* f, const Geo& g, const float* A, const float* B, float
     if (neg || !in_range(f, A, la) || !in_range(f, B, lb) || !in_range(f, C, lc)) return hipErrorInvalidValue;
     GemmArgs a{g.M, g.N, g.K, A, g.am, g.ak, B, g.bk, g.bn, C, g.cm, g.cn,
                g.nb1, g.nb2, g.a1, g.a2, g.b1, g.b2, g.c1, g.c2, alpha, beta, g.ksplit};
+    // with split-K the nb2 slices share the K products of one batch entry
+    const double nb = (double)g.nb1 * (g.ksplit > 0 ? 1 : g.nb2), nc = (double)g.nb1 * g.nb2;
+    f->work.flops += 2.0 * g.M * (double)g.N * g.K * nb;
+    f->work.bytes += 4.0 * (((double)g.M * g.K + (double)g.K * g.N) * nb + (double)g.M * g.N * nc * (beta != 0.f ? 2 : 1));
+    f->work.launches += 1;
     return gemm_bf16(a, st);
 }
 
@@ -258,7 +266,7 @@ int pointwise_dw(ehd_fno* f, int B, int cin, int cout, const float* X, const flo
     g.nb1 = B; g.nb2 = f->nsplit;
     g.ksplit = f->ksplit;
     FCHK(rgemm(f, g, X, dC, f->wscr, 1.f, 0.f, f->st));
-    FCHK(sum_slices(f->wscr, dW, mn, B * f->nsplit, 0.f, f->st));
+    FCHK(sum_slices(f->wscr, dW, mn, B * f->nsplit, 0.f, f->red, f->st));
     return EHD_OK;
 }
 
@@ -448,18 +456,18 @@ int backward(ehd_fno* f, int B)
     const long long ZP = f->ZP;
     int rc;
     if ((rc = pointwise_dw(f, B, H, O, f->h, f->dout, f->g + f->oQ2w))) return rc;
-    FCHK(channel_sum(f->dout, f->g + f->oQ2b, B, O, ZP, false, f->st));
+    FCHK(channel_sum(f->dout, f->g + f->oQ2b, B, O, ZP, false, f->red, f->st));
     if ((rc = pointwise_dx(f, B, H, O, f->p + f->oQ2w, f->dout, f->dh, 0.f))) return rc;
     FCHK(gelu_bwd(f->dh, f->z1, (long long)B * H * ZP, f->st));
     if ((rc = pointwise_dw(f, B, w, H, f->v[L], f->dh, f->g + f->oQ1w))) return rc;
-    FCHK(channel_sum(f->dh, f->g + f->oQ1b, B, H, ZP, false, f->st));
+    FCHK(channel_sum(f->dh, f->g + f->oQ1b, B, H, ZP, false, f->red, f->st));
     float* dv = f->dv0;
     float* dn = f->dv1;
     if ((rc = pointwise_dx(f, B, w, H, f->p + f->oQ1w, f->dh, dv, 0.f))) return rc;
     for (int l = L - 1; l >= 0; --l) {
         if (l < L - 1) FCHK(gelu_bwd(dv, f->u[l], (long long)B * w * ZP, f->st));
         if ((rc = pointwise_dw(f, B, w, w, f->v[l], dv, f->g + f->oW[l]))) return rc;
-        FCHK(channel_sum(dv, f->g + f->ob[l], B, w, ZP, false, f->st));
+        FCHK(channel_sum(dv, f->g + f->ob[l], B, w, ZP, false, f->red, f->st));
         if ((rc = spectral_bwd(f, l, B, dv, dn, 0.f))) return rc;
         if ((rc = pointwise_dx(f, B, w, w, f->p + f->oW[l], dv, dn, 1.f))) return rc;
         float* t = dv;
@@ -467,15 +475,21 @@ int backward(ehd_fno* f, int B)
         dn = t;
     }
     if ((rc = pointwise_dw(f, B, f->c.d_in, w, f->in, dv, f->g + f->oPw))) return rc;
-    FCHK(channel_sum(dv, f->g + f->oPb, B, w, ZP, false, f->st));
+    FCHK(channel_sum(dv, f->g + f->oPb, B, w, ZP, false, f->red, f->st));
     return EHD_OK;
 }
 
 int loss_and_grad(ehd_fno* f, int B, const float* in, const float* target, double* loss)
 {
     const long long nin = (long long)B * f->c.d_in * f->ZP, nout = (long long)B * f->c.d_out * f->ZP;
-    HIPCHK(hipMemcpyAsync(f->in, in, nin * sizeof(float), hipMemcpyHostToDevice, f->st));
-    HIPCHK(hipMemcpyAsync(f->tgt, target, nout * sizeof(float), hipMemcpyHostToDevice, f->st));
+    if (in || target) {
+        if (!in || !target) return set_error(EHD_EINVAL, "inputs and targets go together (or both NULL)");
+        HIPCHK(hipMemcpyAsync(f->in, in, nin * sizeof(float), hipMemcpyHostToDevice, f->st));
+        HIPCHK(hipMemcpyAsync(f->tgt, target, nout * sizeof(float), hipMemcpyHostToDevice, f->st));
+        f->resident = B;
+    } else if (B > f->resident) {
+        return set_error(EHD_EINVAL, "no resident batch of %d samples (last loaded: %d)", B, f->resident);
+    }
     int rc = forward(f, B);
     if (rc) return rc;
     FCHK(mse(f->out, f->tgt, f->dout, f->part, nout, f->st));
@@ -573,6 +587,9 @@ int ehd_fno_create(const ehd_fno_config* cfg, ehd_fno** out)
                                                     (long long)c.d_hidden * c.d_out});
         f->wscr_n = B * f->nsplit * wmax;
         if ((rc = falloc(f, &f->wscr, f->wscr_n))) return bail(rc);
+        const int cmax = std::max({c.width, c.d_hidden, c.d_out});
+        const long long nred = std::max(sum_slices_part_floats(wmax), channel_sum_part_floats((int)B, cmax, f->ZP));
+        if ((rc = falloc(f, &f->red, nred))) return bail(rc);
     }
     {
         void* q = nullptr;
@@ -622,6 +639,7 @@ int ehd_fno_forward(ehd_fno* f, int batch, const float* in, float* out)
     int rc = check_batch(f, batch);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(f->in, in, (size_t)batch * f->c.d_in * f->ZP * sizeof(float), hipMemcpyHostToDevice, f->st));
+    f->resident = 0;   // the inputs no longer match the resident targets
     if ((rc = forward(f, batch))) return rc;
     HIPCHK(hipMemcpyAsync(out, f->out, (size_t)batch * f->c.d_out * f->ZP * sizeof(float), hipMemcpyDeviceToHost, f->st));
     HIPCHK(hipStreamSynchronize(f->st));
@@ -630,7 +648,7 @@ int ehd_fno_forward(ehd_fno* f, int batch, const float* in, float* out)
 
 int ehd_fno_gradients(ehd_fno* f, int batch, const float* in, const float* target, double* loss, float* grad)
 {
-    if (!f || !in || !target) return set_error(EHD_EINVAL, "null argument");
+    if (!f) return set_error(EHD_EINVAL, "null argument");
     int rc = check_batch(f, batch);
     if (rc) return rc;
     if ((rc = loss_and_grad(f, batch, in, target, loss))) return rc;
@@ -640,13 +658,60 @@ int ehd_fno_gradients(ehd_fno* f, int batch, const float* in, const float* targe
 
 int ehd_fno_train_step(ehd_fno* f, int batch, const float* in, const float* target, float lr, double* loss)
 {
-    if (!f || !in || !target) return set_error(EHD_EINVAL, "null argument");
+    if (!f) return set_error(EHD_EINVAL, "null argument");
     int rc = check_batch(f, batch);
     if (rc) return rc;
     if ((rc = loss_and_grad(f, batch, in, target, loss))) return rc;
     f->adam_t += 1;
     FCHK(adam(f->p, f->g, f->m1, f->m2, f->nparam, lr, 0.9f, 0.999f, 1e-8f, f->adam_t, f->st));
     HIPCHK(hipStreamSynchronize(f->st));
+    return EHD_OK;
+}
+
+int ehd_fno_gemm_work(ehd_fno* f, int reset, ehd_fno_work* out)
+{
+    if (!f || !out) return set_error(EHD_EINVAL, "null argument");
+    *out = f->work;
+    if (reset) f->work = ehd_fno_work{};
+    return EHD_OK;
+}
+
+int ehd_fno_gemm_strided(int M, int N, int K, int nb, int ksplit, const float* A, const long long as[3],
+                         long long a_len, const float* B, const long long bs[3], long long b_len, float* C,
+                         const long long cs[4], long long c_len, float alpha, float beta)
+{
+    if (M < 1 || N < 1 || K < 1 || nb < 1 || ksplit < 0 || !A || !B || !C || !as || !bs || !cs)
+        return set_error(EHD_EINVAL, "bad GEMM arguments");
+    for (int i = 0; i < 3; ++i)
+        if (as[i] < 0 || bs[i] < 0 || cs[i] < 0 || cs[3] < 0) return set_error(EHD_EINVAL, "negative stride");
+    const int ns = ksplit > 0 ? (K + ksplit - 1) / ksplit : 1;
+    const long long la = (nb - 1) * as[2] + (M - 1) * as[0] + (long long)(K - 1) * as[1];
+    const long long lb = (nb - 1) * bs[2] + (long long)(K - 1) * bs[0] + (N - 1) * bs[1];
+    const long long lc = (nb - 1) * cs[2] + (ns - 1) * cs[3] + (M - 1) * cs[0] + (N - 1) * cs[1];
+    if (la >= a_len || lb >= b_len || lc >= c_len) return set_error(EHD_EINVAL, "strides reach outside the arrays");
+    float *dA = nullptr, *dB = nullptr, *dC = nullptr;
+    auto cleanup = [&]() {
+        if (dA) (void)hipFree(dA);
+        if (dB) (void)hipFree(dB);
+        if (dC) (void)hipFree(dC);
+    };
+    if (hipMalloc(&dA, (size_t)a_len * 4) != hipSuccess || hipMalloc(&dB, (size_t)b_len * 4) != hipSuccess ||
+        hipMalloc(&dC, (size_t)c_len * 4) != hipSuccess) {
+        cleanup();
+        return set_error(EHD_ENOMEM, "GEMM allocation failed");
+    }
+    hipError_t e = hipMemcpy(dA, A, (size_t)a_len * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dB, B, (size_t)b_len * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dC, C, (size_t)c_len * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        GemmArgs g{M, N, K, dA, as[0], as[1], dB, bs[0], bs[1], dC, cs[0], cs[1], nb, ns,
+                   as[2], 0, bs[2], 0, cs[2], cs[3], alpha, beta, ksplit};
+        e = gemm_bf16(g, nullptr);
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(C, dC, (size_t)c_len * 4, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return set_error(EHD_EHIP, "GEMM failed: %s", hipGetErrorString(e));
     return EHD_OK;
 }
 

@@ -23,14 +23,22 @@ struct GemmArgs {
 };
 
 hipError_t gemm_bf16(const GemmArgs& g, hipStream_t st);
-// dst[i] = beta*dst[i] + sum_j src[j*n + i], j < nslices
-hipError_t sum_slices(const float* src, float* dst, long long n, int nslices, float beta, hipStream_t st);
+// staging layout gemm_bf16 picks for A and B: 0 k unit-stride, 1 m (n) unit-stride
+// (16-byte loads), 2 element loads
+int gemm_layout_a(const GemmArgs& g);
+int gemm_layout_b(const GemmArgs& g);
+// dst[i] = beta*dst[i] + sum_j src[j*n + i], j < nslices; part holds
+// sum_slices_part_floats(n) floats of workspace
+long long sum_slices_part_floats(long long n);
+hipError_t sum_slices(const float* src, float* dst, long long n, int nslices, float beta, float* part, hipStream_t st);
 // y = x + bias[c] over [B][C][S]; with gelu: u keeps the pre-activation, y = gelu(u)
 hipError_t bias_act(float* y, float* u, const float* bias, int B, int C, long long S, bool gelu, hipStream_t st);
 // g *= gelu'(u)
 hipError_t gelu_bwd(float* g, const float* u, long long n, hipStream_t st);
-// db[c] (+)= sum_{b,s} g[b][c][s]
-hipError_t channel_sum(const float* g, float* db, int B, int C, long long S, bool accumulate, hipStream_t st);
+// db[c] (+)= sum_{b,s} g[b][c][s]; part holds channel_sum_part_floats(B, C, S) floats of workspace
+long long channel_sum_part_floats(int B, int C, long long S);
+hipError_t channel_sum(const float* g, float* db, int B, int C, long long S, bool accumulate, float* part,
+                       hipStream_t st);
 // grad = 2 (out - tgt) / n, part[mse_blocks()] = partial sums of (out - tgt)^2
 int mse_blocks();
 hipError_t mse(const float* out, const float* tgt, float* grad, double* part, long long n, hipStream_t st);

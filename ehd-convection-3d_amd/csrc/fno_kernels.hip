@@ -1,13 +1,15 @@
 // fno_kernels.hip -- the surrogate's GPU kernels: a strided, batched GEMM on
-// the bf16 matrix cores (v_mfma_f32_16x16x16_bf16, FP32 accumulation) that
-// carries every contraction of the Fourier neural operator (the truncated
-// DFTs written as matrices, the per-mode channel mixing, the pointwise
-// layers and all their adjoints), plus the elementwise kernels (bias, GELU,
-// loss, column sums, Adam).
+// gfx950's bf16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulation)
+// that carries every contraction of the Fourier neural operator (the
+// truncated DFTs written as matrices, the per-mode channel mixing, the
+// pointwise layers and all their adjoints), plus the elementwise kernels
+// (bias, GELU, loss, column sums, Adam).
 //
 // Operands are FP32 in memory and rounded to bf16 (nearest even) when staged
 // into LDS; accumulation and the epilogue are FP32.  Any element strides are
-// allowed, so transposes, real/imaginary planes and sub-blocks cost no copy.
+// allowed, so transposes, real/imaginary planes and sub-blocks cost no copy;
+// an operand whose k or m/n axis is unit-stride (and 16-byte aligned) is
+// staged with 16-byte loads, anything else element by element.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -17,25 +19,114 @@
 namespace ehd {
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ short to_bf16(float f)
+__device__ __forceinline__ uint32_t to_bf16(float f)
 {
     uint32_t u = __float_as_uint(f);
     u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even (finite inputs)
-    return (short)(u >> 16);
+    return u >> 16;
 }
 
-constexpr int TM = 64, TN = 64, TK = 16;
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) { return to_bf16(lo) | (to_bf16(hi) << 16); }
 
-// Workgroup: 256 threads = 4 waves; the 64x64 C tile is split 2x2 over the
-// waves (32x32 each = 2x2 MFMA blocks of 16x16).  Per K step of 16 the A
-// tile (64x16) and B tile (16x64) are staged in LDS as bf16.
+// Tile 64 x 64 x 32; 256 threads = 4 waves, each owning a 32 x 32 block of C
+// (2 x 2 MFMA blocks of 16 x 16).  Both operands sit in LDS as bf16 rows of
+// k (A as [m][k], B as [n][k]), so every lane fetches its 8 k-values of an
+// MFMA operand with one 16-byte LDS read; the row pitch of 40 elements (80
+// bytes) keeps those reads 16-byte aligned and spreads the 16 rows of an
+// operand over the banks.  Two LDS buffers: the global loads of k-step s+1
+// are in flight while the matrix cores work on step s.
+constexpr int TM = 64, TN = 64, TK = 32, LP = TK + 8;
+constexpr int kSliceGroups = 32;     // first-pass groups of slices in sum_slices
+constexpr int kRowChunk = 8192;      // row elements per workgroup in channel_sum
+constexpr int kActIter = 4;          // float4s per thread in bias_act
+
+// operand staging layouts
+constexpr int L_K = 0;       // k unit-stride: 16-byte loads along k
+constexpr int L_MN = 1;      // m (A) or n (B) unit-stride: 16-byte loads along m / n
+constexpr int L_ANY = 2;     // element loads
+
+// One operand tile (64 rows of m or n, 32 of k) in flight: 8 floats per thread.
+struct Stage {
+    float v[8];
+};
+
+// P[r][k] at P + r*sr + k*sk for r < R, k in [k0, ke): rows r0.. of the tile
+template <int L>
+__device__ __forceinline__ void load_tile(Stage& st, const float* __restrict__ P, long long sr, long long sk, int r0,
+                                          int R, int k0, int ke, int tid)
+{
+    if constexpr (L == L_K) {
+        // 64 rows x 8 quads of k; thread: quads tid and tid + 256
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int q = tid + 256 * h, r = q >> 3, k = k0 + 4 * (q & 7);
+            float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r0 + r < R && k < ke) x = *(const float4*)(P + (long long)(r0 + r) * sr + k);
+            st.v[4 * h] = x.x;
+            st.v[4 * h + 1] = x.y;
+            st.v[4 * h + 2] = x.z;
+            st.v[4 * h + 3] = x.w;
+        }
+    } else if constexpr (L == L_MN) {
+        // 16 quads of rows x 16 pairs of k; thread: one quad, k = 2p and 2p+1
+        const int rq = 4 * (tid & 15), k = k0 + 2 * (tid >> 4);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r0 + rq < R && k + h < ke) x = *(const float4*)(P + (r0 + rq) + (long long)(k + h) * sk);
+            st.v[4 * h] = x.x;
+            st.v[4 * h + 1] = x.y;
+            st.v[4 * h + 2] = x.z;
+            st.v[4 * h + 3] = x.w;
+        }
+    } else {
+        // element loads, the smaller stride fastest across the threads
+        const bool kfast = sk <= sr;
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            const int e = tid + 256 * h;
+            const int r = kfast ? (e >> 5) : (e & 63), k = k0 + (kfast ? (e & 31) : (e >> 6));
+            st.v[h] = (r0 + r < R && k < ke) ? P[(long long)(r0 + r) * sr + (long long)k * sk] : 0.f;
+        }
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void store_tile(const Stage& st, unsigned short (*T)[LP], int tid, long long sr,
+                                           long long sk)
+{
+    if constexpr (L == L_K) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int q = tid + 256 * h, r = q >> 3, k = 4 * (q & 7);
+            uint2 w;
+            w.x = pack_bf16(st.v[4 * h], st.v[4 * h + 1]);
+            w.y = pack_bf16(st.v[4 * h + 2], st.v[4 * h + 3]);
+            *(uint2*)&T[r][k] = w;
+        }
+    } else if constexpr (L == L_MN) {
+        const int rq = 4 * (tid & 15), k = 2 * (tid >> 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) *(uint32_t*)&T[rq + e][k] = pack_bf16(st.v[e], st.v[4 + e]);
+    } else {
+        const bool kfast = sk <= sr;
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            const int e = tid + 256 * h;
+            const int r = kfast ? (e >> 5) : (e & 63), k = kfast ? (e & 31) : (e >> 6);
+            T[r][k] = (unsigned short)to_bf16(st.v[h]);
+        }
+    }
+}
+
+template <int AL, int BL>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
 {
-    __shared__ short As[TM][TK + 2];
-    __shared__ short Bs[TK][TN + 2];
+    __shared__ __align__(16) unsigned short As[2][TM][LP];
+    __shared__ __align__(16) unsigned short Bs[2][TN][LP];
     const int bz = blockIdx.z;
     const int i1 = bz / g.nb2, i2 = bz - i1 * g.nb2;
     const float* __restrict__ A = g.A + i1 * g.a1 + i2 * g.a2;
@@ -52,39 +143,38 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
 
     const int kb = g.ksplit > 0 ? i2 * g.ksplit : 0;
     const int ke = g.ksplit > 0 ? min(g.K, kb + g.ksplit) : g.K;
+    Stage sa, sb;
+    load_tile<AL>(sa, A, g.am, g.ak, m0, g.M, kb, ke, tid);
+    load_tile<BL>(sb, B, g.bn, g.bk, n0, g.N, kb, ke, tid);
+    store_tile<AL>(sa, As[0], tid, g.am, g.ak);
+    store_tile<BL>(sb, Bs[0], tid, g.bn, g.bk);
+    __syncthreads();
+    // MFMA 16x16x32 bf16 operands: A[i][k] in lane i + 16*(k/8), element k%8;
+    // B[k][j] in lane j + 16*(k/8), element k%8
+    const int li = lane & 15, lk = (lane >> 4) * 8;
+    int buf = 0;
     for (int k0 = kb; k0 < ke; k0 += TK) {
-        // stage: 64*16 = 1024 elements of A and of B, 4 per thread
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int e = tid + 256 * r;
-            const int am = e / TK, ak = e - am * TK;          // A tile row-major [64][16]
-            const int gm = m0 + am, gk = k0 + ak;
-            As[am][ak] = (gm < g.M && gk < ke) ? to_bf16(A[(long long)gm * g.am + (long long)gk * g.ak]) : (short)0;
-            const int bk = e / TN, bn = e - bk * TN;          // B tile row-major [16][64]
-            const int hk = k0 + bk, hn = n0 + bn;
-            Bs[bk][bn] = (hk < ke && hn < g.N) ? to_bf16(B[(long long)hk * g.bk + (long long)hn * g.bn]) : (short)0;
+        const bool more = k0 + TK < ke;
+        if (more) {
+            load_tile<AL>(sa, A, g.am, g.ak, m0, g.M, k0 + TK, ke, tid);
+            load_tile<BL>(sb, B, g.bn, g.bk, n0, g.N, k0 + TK, ke, tid);
         }
-        __syncthreads();
-        // MFMA 16x16x16 bf16 operand layout: A[i][k] in lane i + 16*(k/4),
-        // element k%4; B[k][j] in lane j + 16*(k/4), element k%4.
-        const int li = lane & 15, lk = (lane >> 4) * 4;
-        bf16x4 af[2], bfr[2];
+        bf16x8 af[2], bfr[2];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            const int row = wm + a * 16 + li;
-            af[a] = bf16x4{As[row][lk], As[row][lk + 1], As[row][lk + 2], As[row][lk + 3]};
-        }
+        for (int a = 0; a < 2; ++a) __builtin_memcpy(&af[a], &As[buf][wm + a * 16 + li][lk], 16);
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int col = wn + b * 16 + li;
-            bfr[b] = bf16x4{Bs[lk][col], Bs[lk + 1][col], Bs[lk + 2][col], Bs[lk + 3][col]};
-        }
+        for (int b = 0; b < 2; ++b) __builtin_memcpy(&bfr[b], &Bs[buf][wn + b * 16 + li][lk], 16);
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfr[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        if (more) {
+            store_tile<AL>(sa, As[buf ^ 1], tid, g.am, g.ak);
+            store_tile<BL>(sb, Bs[buf ^ 1], tid, g.bn, g.bk);
+        }
         __syncthreads();
+        buf ^= 1;
     }
     // D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4
     const int lj = lane & 15, li4 = (lane >> 4) * 4;
@@ -103,64 +193,151 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
             }
 }
 
-__global__ void __launch_bounds__(256) sum_slices_kernel(const float* __restrict__ src, float* __restrict__ dst,
-                                                         long long n, int nslices, float beta)
-{
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        float s = 0.f;
-        for (int j = 0; j < nslices; ++j) s += src[(long long)j * n + i];
-        dst[i] = (beta != 0.f) ? beta * dst[i] + s : s;
-    }
-}
+// Sums of many slices and of channel rows, deterministic (fixed order, no
+// atomics) and spread over enough workgroups to fill the chip: a first pass
+// writes partial sums per group of slices / chunk of a row, a second adds
+// the partials in a fixed order.
 
-// y[b][c][s] = x[b][c][s] + bias[c]; act: u = that, y = gelu(u)
-__global__ void __launch_bounds__(256) bias_act_kernel(float* __restrict__ y, float* __restrict__ u,
-                                                       const float* __restrict__ bias, int C, long long S,
-                                                       long long total, int act)
+// 256 threads = 64 consecutive elements x 4 slice lanes; one workgroup per
+// (64 elements, group of slices): part[g][i] = sum of src[j*n + i] over the
+// slices j of group g
+__global__ void __launch_bounds__(256) slice_group_sum_kernel(const float* __restrict__ src, float* __restrict__ part,
+                                                              long long n, int nslices)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (long long)gridDim.x * blockDim.x) {
-        const int c = (int)((i / S) % C);
-        const float v = y[i] + bias[c];
-        if (act) {
-            u[i] = v;
-            y[i] = 0.5f * v * (1.0f + erff(v * 0.70710678118654752f));
-        } else {
-            y[i] = v;
-        }
-    }
-}
-
-// g[i] *= gelu'(u[i])
-__global__ void __launch_bounds__(256) gelu_bwd_kernel(float* __restrict__ g, const float* __restrict__ u,
-                                                       long long total)
-{
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (long long)gridDim.x * blockDim.x) {
-        const float x = u[i];
-        const float d = 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * expf(-0.5f * x * x) * 0.3989422804014327f;
-        g[i] *= d;
-    }
-}
-
-// db[c] (+)= sum over b, s of g[b][c][s]: one workgroup per channel
-__global__ void __launch_bounds__(256) channel_sum_kernel(const float* __restrict__ g, float* __restrict__ db, int B,
-                                                          int C, long long S, int accumulate)
-{
-    __shared__ float red[256];
-    const int c = blockIdx.x;
+    __shared__ float red[4][64];
+    const int e = threadIdx.x & 63, ls = threadIdx.x >> 6, g = blockIdx.y;
+    const long long i = (long long)blockIdx.x * 64 + e;
+    const int per = (nslices + kSliceGroups - 1) / kSliceGroups;
+    const int j0 = g * per, j1 = min(nslices, j0 + per);
     float s = 0.f;
-    for (int b = 0; b < B; ++b) {
-        const float* p = g + ((long long)b * C + c) * S;
-        for (long long i = threadIdx.x; i < S; i += 256) s += p[i];
-    }
+    if (i < n)
+        for (int j = j0 + ls; j < j1; j += 4) s += src[(long long)j * n + i];
+    red[ls][e] = s;
+    __syncthreads();
+    if (ls == 0 && i < n) part[(long long)g * n + i] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+}
+
+__global__ void __launch_bounds__(256) group_total_kernel(const float* __restrict__ part, float* __restrict__ dst,
+                                                          long long n, float beta)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int g = 0; g < kSliceGroups; ++g) s += part[(long long)g * n + i];
+    dst[i] = (beta != 0.f) ? beta * dst[i] + s : s;
+}
+
+__device__ __forceinline__ float block_sum_256(float s, float* red)
+{
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[c] = accumulate ? db[c] + red[0] : red[0];
+    return red[0];
+}
+
+// part[row][chunk] = sum of g[row][chunk*kRowChunk ...) for the rows (b, c)
+template <bool VEC>
+__global__ void __launch_bounds__(256) row_chunk_sum_kernel(const float* __restrict__ g, float* __restrict__ part,
+                                                            long long S, int nchunk)
+{
+    __shared__ float red[256];
+    const long long row = blockIdx.y;
+    const float* __restrict__ p = g + row * S;
+    const long long s0 = (long long)blockIdx.x * kRowChunk, s1 = min(S, s0 + kRowChunk);
+    float s = 0.f;
+    if constexpr (VEC) {
+        for (long long i = s0 + 4 * threadIdx.x; i < s1; i += 1024) {
+            const float4 v = *(const float4*)(p + i);
+            s += (v.x + v.y) + (v.z + v.w);
+        }
+    } else {
+        for (long long i = s0 + threadIdx.x; i < s1; i += 256) s += p[i];
+    }
+    const float t = block_sum_256(s, red);
+    if (threadIdx.x == 0) part[row * nchunk + blockIdx.x] = t;
+}
+
+// db[c] (+)= sum over b and chunks of part[(b C + c) nchunk + chunk]: one workgroup per channel
+__global__ void __launch_bounds__(256) channel_total_kernel(const float* __restrict__ part, float* __restrict__ db,
+                                                            int B, int C, int nchunk, int accumulate)
+{
+    __shared__ float red[256];
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (int t = threadIdx.x; t < B * nchunk; t += 256) {
+        const int b = t / nchunk, k = t - b * nchunk;
+        s += part[((long long)b * C + c) * nchunk + k];
+    }
+    const float tot = block_sum_256(s, red);
+    if (threadIdx.x == 0) db[c] = accumulate ? db[c] + tot : tot;
+}
+
+__device__ __forceinline__ float gelu(float v) { return 0.5f * v * (1.0f + erff(v * 0.70710678118654752f)); }
+
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * expf(-0.5f * x * x) * 0.3989422804014327f;
+}
+
+// y[row][s] = x[row][s] + bias[row % C]; act: u = that, y = gelu(u).  One row
+// (b, c) per blockIdx.y, so the bias is one value per workgroup; each thread
+// takes kActIter float4s (VEC) or elements of the row.
+template <bool VEC>
+__global__ void __launch_bounds__(256) bias_act_kernel(float* __restrict__ y, float* __restrict__ u,
+                                                       const float* __restrict__ bias, int C, long long S, int act)
+{
+    const long long row = blockIdx.y;
+    const float bv = bias[row % C];
+    float* __restrict__ yr = y + row * S;
+    float* __restrict__ ur = act ? u + row * S : nullptr;
+    if constexpr (VEC) {
+        const long long base = (long long)blockIdx.x * (1024 * kActIter) + 4 * threadIdx.x;
+#pragma unroll
+        for (int r = 0; r < kActIter; ++r) {
+            const long long i = base + 1024 * r;
+            if (i >= S) break;
+            float4 v = *(float4*)(yr + i);
+            v.x += bv; v.y += bv; v.z += bv; v.w += bv;
+            if (act) {
+                *(float4*)(ur + i) = v;
+                v = make_float4(gelu(v.x), gelu(v.y), gelu(v.z), gelu(v.w));
+            }
+            *(float4*)(yr + i) = v;
+        }
+    } else {
+        const long long base = (long long)blockIdx.x * (1024 * kActIter) + threadIdx.x;
+        for (int r = 0; r < 4 * kActIter; ++r) {
+            const long long i = base + 256 * r;
+            if (i >= S) break;
+            const float v = yr[i] + bv;
+            if (act) {
+                ur[i] = v;
+                yr[i] = gelu(v);
+            } else {
+                yr[i] = v;
+            }
+        }
+    }
+}
+
+// g[i] *= gelu'(u[i]); VEC: float4 i in [0, n/4)
+template <bool VEC>
+__global__ void __launch_bounds__(256) gelu_bwd_kernel(float* __restrict__ g, const float* __restrict__ u, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    if constexpr (VEC) {
+        for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n / 4; i += stride) {
+            const float4 x = ((const float4*)u)[i];
+            float4 v = ((float4*)g)[i];
+            v.x *= gelu_grad(x.x); v.y *= gelu_grad(x.y); v.z *= gelu_grad(x.z); v.w *= gelu_grad(x.w);
+            ((float4*)g)[i] = v;
+        }
+    } else {
+        for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) g[i] *= gelu_grad(u[i]);
+    }
 }
 
 // d = out - target; grad = 2 d / n; partial sums of d^2 per workgroup
@@ -204,39 +381,102 @@ unsigned grid_of(long long n)
     return (unsigned)(g < 65536 ? (g < 1 ? 1 : g) : 65536);
 }
 
+bool aligned16(const float* p) { return ((uintptr_t)p & 15u) == 0; }
+bool mul4(long long v) { return (v & 3) == 0; }
+
 }  // namespace
+
+// Staging layout of A: 16-byte loads need the unit-stride axis in quads that
+// lie wholly inside or outside the matrix (extent a multiple of 4) and every
+// other offset, batch strides included, a multiple of 4 floats from a 16-byte
+// aligned base.
+int gemm_layout_a(const GemmArgs& g)
+{
+    const bool base = aligned16(g.A) && mul4(g.a1) && mul4(g.a2);
+    if (base && g.ak == 1 && mul4(g.am) && mul4(g.K) && mul4(g.ksplit)) return L_K;
+    if (base && g.am == 1 && mul4(g.ak) && mul4(g.M)) return L_MN;
+    return L_ANY;
+}
+
+int gemm_layout_b(const GemmArgs& g)
+{
+    const bool base = aligned16(g.B) && mul4(g.b1) && mul4(g.b2);
+    if (base && g.bk == 1 && mul4(g.bn) && mul4(g.K) && mul4(g.ksplit)) return L_K;
+    if (base && g.bn == 1 && mul4(g.bk) && mul4(g.N)) return L_MN;
+    return L_ANY;
+}
 
 hipError_t gemm_bf16(const GemmArgs& g, hipStream_t st)
 {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.nb1 <= 0 || g.nb2 <= 0) return hipErrorInvalidValue;
     dim3 grid((g.M + TM - 1) / TM, (g.N + TN - 1) / TN, g.nb1 * g.nb2);
     if (grid.y > 65535 || grid.z > 65535) return hipErrorInvalidValue;
-    gemm_bf16_kernel<<<grid, 256, 0, st>>>(g);
+    const int al = gemm_layout_a(g), bl = gemm_layout_b(g);
+    switch (al * 3 + bl) {
+    case 0: gemm_bf16_kernel<L_K, L_K><<<grid, 256, 0, st>>>(g); break;
+    case 1: gemm_bf16_kernel<L_K, L_MN><<<grid, 256, 0, st>>>(g); break;
+    case 2: gemm_bf16_kernel<L_K, L_ANY><<<grid, 256, 0, st>>>(g); break;
+    case 3: gemm_bf16_kernel<L_MN, L_K><<<grid, 256, 0, st>>>(g); break;
+    case 4: gemm_bf16_kernel<L_MN, L_MN><<<grid, 256, 0, st>>>(g); break;
+    case 5: gemm_bf16_kernel<L_MN, L_ANY><<<grid, 256, 0, st>>>(g); break;
+    case 6: gemm_bf16_kernel<L_ANY, L_K><<<grid, 256, 0, st>>>(g); break;
+    case 7: gemm_bf16_kernel<L_ANY, L_MN><<<grid, 256, 0, st>>>(g); break;
+    default: gemm_bf16_kernel<L_ANY, L_ANY><<<grid, 256, 0, st>>>(g); break;
+    }
     return hipGetLastError();
 }
 
-hipError_t sum_slices(const float* src, float* dst, long long n, int nslices, float beta, hipStream_t st)
+long long sum_slices_part_floats(long long n) { return (long long)kSliceGroups * n; }
+
+hipError_t sum_slices(const float* src, float* dst, long long n, int nslices, float beta, float* part, hipStream_t st)
 {
-    sum_slices_kernel<<<grid_of(n), 256, 0, st>>>(src, dst, n, nslices, beta);
+    if (n <= 0 || nslices <= 0) return hipErrorInvalidValue;
+    const long long nb = (n + 63) / 64;
+    if (nb > 2147483647LL) return hipErrorInvalidValue;
+    slice_group_sum_kernel<<<dim3((unsigned)nb, kSliceGroups), 256, 0, st>>>(src, part, n, nslices);
+    group_total_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(part, dst, n, beta);
     return hipGetLastError();
 }
 
 hipError_t bias_act(float* y, float* u, const float* bias, int B, int C, long long S, bool gelu, hipStream_t st)
 {
-    const long long total = (long long)B * C * S;
-    bias_act_kernel<<<grid_of(total), 256, 0, st>>>(y, u, bias, C, S, total, gelu ? 1 : 0);
+    const long long rows = (long long)B * C;
+    if (rows <= 0 || rows > 65535 || S <= 0) return hipErrorInvalidValue;
+    const bool vec = (S % 4 == 0) && ((uintptr_t)y % 16 == 0) && (!gelu || (uintptr_t)u % 16 == 0);
+    const dim3 grid((unsigned)((S + 1024 * kActIter - 1) / (1024 * kActIter)), (unsigned)rows);
+    if (vec)
+        bias_act_kernel<true><<<grid, 256, 0, st>>>(y, u, bias, C, S, gelu ? 1 : 0);
+    else
+        bias_act_kernel<false><<<grid, 256, 0, st>>>(y, u, bias, C, S, gelu ? 1 : 0);
     return hipGetLastError();
 }
 
 hipError_t gelu_bwd(float* g, const float* u, long long n, hipStream_t st)
 {
-    gelu_bwd_kernel<<<grid_of(n), 256, 0, st>>>(g, u, n);
+    if (n % 4 == 0 && (uintptr_t)g % 16 == 0 && (uintptr_t)u % 16 == 0)
+        gelu_bwd_kernel<true><<<grid_of(n / 4), 256, 0, st>>>(g, u, n);
+    else
+        gelu_bwd_kernel<false><<<grid_of(n), 256, 0, st>>>(g, u, n);
     return hipGetLastError();
 }
 
-hipError_t channel_sum(const float* g, float* db, int B, int C, long long S, bool accumulate, hipStream_t st)
+long long channel_sum_part_floats(int B, int C, long long S)
 {
-    channel_sum_kernel<<<C, 256, 0, st>>>(g, db, B, C, S, accumulate ? 1 : 0);
+    return (long long)B * C * ((S + kRowChunk - 1) / kRowChunk);
+}
+
+hipError_t channel_sum(const float* g, float* db, int B, int C, long long S, bool accumulate, float* part,
+                       hipStream_t st)
+{
+    const long long rows = (long long)B * C;
+    if (rows <= 0 || rows > 65535 || S <= 0) return hipErrorInvalidValue;
+    const int nchunk = (int)((S + kRowChunk - 1) / kRowChunk);
+    const dim3 grid((unsigned)nchunk, (unsigned)rows);
+    if (S % 4 == 0 && (uintptr_t)g % 16 == 0)
+        row_chunk_sum_kernel<true><<<grid, 256, 0, st>>>(g, part, S, nchunk);
+    else
+        row_chunk_sum_kernel<false><<<grid, 256, 0, st>>>(g, part, S, nchunk);
+    channel_total_kernel<<<C, 256, 0, st>>>(part, db, B, C, nchunk, accumulate ? 1 : 0);
     return hipGetLastError();
 }
 

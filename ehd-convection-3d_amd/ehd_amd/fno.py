@@ -16,8 +16,13 @@ class Config(C.Structure):
                                         "d_out", "max_batch")]
 
 
+class Work(C.Structure):
+    _fields_ = [("flops", C.c_double), ("bytes", C.c_double), ("launches", C.c_longlong)]
+
+
 EXPORTS = ("ehd_fno_create", "ehd_fno_destroy", "ehd_fno_num_params", "ehd_fno_set_params", "ehd_fno_get_params",
-           "ehd_fno_forward", "ehd_fno_gradients", "ehd_fno_train_step", "ehd_fno_gemm")
+           "ehd_fno_forward", "ehd_fno_gradients", "ehd_fno_train_step", "ehd_fno_gemm", "ehd_fno_gemm_strided",
+           "ehd_fno_gemm_work")
 
 _bound = False
 
@@ -40,8 +45,12 @@ def _lib():
         L.ehd_fno_gradients.argtypes = [vp, C.c_int, fp, fp, C.POINTER(C.c_double), fp]
         L.ehd_fno_train_step.argtypes = [vp, C.c_int, fp, fp, C.c_float, C.POINTER(C.c_double)]
         L.ehd_fno_gemm.argtypes = [C.c_int, C.c_int, C.c_int, fp, fp, fp]
+        llp = C.POINTER(C.c_longlong)
+        L.ehd_fno_gemm_strided.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fp, llp, C.c_longlong, fp,
+                                           llp, C.c_longlong, fp, llp, C.c_longlong, C.c_float, C.c_float]
+        L.ehd_fno_gemm_work.argtypes = [vp, C.c_int, C.POINTER(Work)]
         for n in ("ehd_fno_set_params", "ehd_fno_get_params", "ehd_fno_forward", "ehd_fno_gradients",
-                  "ehd_fno_train_step", "ehd_fno_gemm"):
+                  "ehd_fno_train_step", "ehd_fno_gemm", "ehd_fno_gemm_strided", "ehd_fno_gemm_work"):
             getattr(L, n).restype = C.c_int
         _bound = True
     return L
@@ -61,6 +70,20 @@ def gemm(A: np.ndarray, B: np.ndarray) -> np.ndarray:
     Cm = np.empty((M, N), np.float32)
     _check(_lib().ehd_fno_gemm(M, N, K, _fp(A), _fp(B), _fp(Cm)))
     return Cm
+
+
+def gemm_strided(M, N, K, A, a_strides, B, b_strides, C_, c_strides, nb=1, ksplit=0, alpha=1.0, beta=0.0):
+    """The library GEMM on flat host arrays with explicit element strides
+    (ehd_fno_gemm_strided): a_strides = (m, k, batch), b_strides = (k, n,
+    batch), c_strides = (m, n, batch, split-K slice).  C_ is updated in place."""
+    A = np.ascontiguousarray(A, dtype=np.float32).reshape(-1)
+    B = np.ascontiguousarray(B, dtype=np.float32).reshape(-1)
+    assert C_.dtype == np.float32 and C_.flags.c_contiguous
+    Cf = C_.reshape(-1)
+    arr = lambda v: (C.c_longlong * len(v))(*[int(x) for x in v])
+    _check(_lib().ehd_fno_gemm_strided(M, N, K, nb, ksplit, _fp(A), arr(a_strides), A.size, _fp(B), arr(b_strides),
+                                       B.size, _fp(Cf), arr(c_strides), Cf.size, float(alpha), float(beta)))
+    return C_
 
 
 # flat parameter order of include/ehd_fno.h
@@ -140,6 +163,7 @@ class FNO:
         c = self.cfg
         out = np.empty((B, c.d_out, c.Z, c.Y, c.X), np.float32)
         _check(_lib().ehd_fno_forward(self._h, B, _fp(a), _fp(out)))
+        self._resident = 0
         return out
 
     def gradients(self, a, target):
@@ -148,11 +172,25 @@ class FNO:
         g = np.empty(self.n, np.float32)
         loss = C.c_double()
         _check(_lib().ehd_fno_gradients(self._h, a.shape[0], _fp(a), _fp(t), C.byref(loss), _fp(g)))
+        self._resident = a.shape[0]
         return loss.value, g
 
-    def train_step(self, a, target, lr: float) -> float:
+    def gemm_work(self, reset: bool = False) -> Work:
+        """FLOPs, bytes and launches of the GEMMs since the last reset."""
+        w = Work()
+        _check(_lib().ehd_fno_gemm_work(self._h, int(reset), C.byref(w)))
+        return w
+
+    def train_step(self, a, target, lr: float, batch: int = 0) -> float:
+        """One Adam step on (a, target); a and target None: on the batch the
+        previous call left on the device (`batch` samples of it, default all)."""
+        loss = C.c_double()
+        if a is None and target is None:
+            _check(_lib().ehd_fno_train_step(self._h, int(batch or getattr(self, "_resident", 0)), None, None,
+                                             float(lr), C.byref(loss)))
+            return loss.value
         a = np.ascontiguousarray(a, dtype=np.float32)
         t = np.ascontiguousarray(target, dtype=np.float32)
-        loss = C.c_double()
         _check(_lib().ehd_fno_train_step(self._h, a.shape[0], _fp(a), _fp(t), float(lr), C.byref(loss)))
+        self._resident = a.shape[0]
         return loss.value

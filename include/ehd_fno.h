@@ -49,16 +49,45 @@ int ehd_fno_get_params(ehd_fno* f, float* params);
 int ehd_fno_forward(ehd_fno* f, int batch, const float* in, float* out);
 
 /* mean-squared-error loss of network(in) against target and its gradient
- * with respect to every parameter (flat order; grad may be NULL) */
+ * with respect to every parameter (flat order; grad may be NULL).  in and
+ * target both NULL: the batch the last gradients / train_step call copied to
+ * the device is used again (no host transfer; batch must not exceed it, and
+ * an ehd_fno_forward call in between invalidates it). */
 int ehd_fno_gradients(ehd_fno* f, int batch, const float* in, const float* target, double* loss, float* grad);
 
 /* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the loss of the batch;
- * loss (may be NULL) receives the loss before the update */
+ * loss (may be NULL) receives the loss before the update; in and target NULL
+ * as in ehd_fno_gradients */
 int ehd_fno_train_step(ehd_fno* f, int batch, const float* in, const float* target, float lr, double* loss);
 
 /* C = A B for row-major host matrices (M x K, K x N) through the bf16
  * matrix-core GEMM the network uses (checks of its operand layout) */
 int ehd_fno_gemm(int M, int N, int K, const float* A, const float* B, float* C);
+
+/* The same GEMM with the element strides and batching the network's stages
+ * use (checks of every staging layout):
+ *   C(b,s)[m][n] = alpha sum_k A(b)[m][k] B(b)[k][n] + beta C(b,s)[m][n]
+ * with A(b)[m][k] = A[b*as[2] + m*as[0] + k*as[1]], B(b)[k][n] =
+ * B[b*bs[2] + k*bs[0] + n*bs[1]], C(b,s)[m][n] = C[b*cs[2] + s*cs[3] +
+ * m*cs[0] + n*cs[1]] for b < nb; with ksplit > 0, slice s < ceil(K/ksplit)
+ * sums only k in [s*ksplit, (s+1)*ksplit) (split-K partial products).
+ * Host arrays of a_len, b_len, c_len floats; every index the strides reach
+ * must lie inside them (checked; strides >= 0). */
+int ehd_fno_gemm_strided(int M, int N, int K, int nb, int ksplit, const float* A, const long long as[3],
+                         long long a_len, const float* B, const long long bs[3], long long b_len, float* C,
+                         const long long cs[4], long long c_len, float alpha, float beta);
+
+/* Work of the GEMMs issued since the last reset (forward, gradients and
+ * training steps): algorithmic FLOPs (2 M N K per real product; a complex
+ * product counts as its four real ones), algorithmic HBM bytes (A and B read
+ * once, C written once and read too when accumulated, all FP32), launches.
+ * reset != 0 clears the counters after reading. */
+typedef struct ehd_fno_work {
+    double flops;
+    double bytes;
+    long long launches;
+} ehd_fno_work;
+int ehd_fno_gemm_work(ehd_fno* f, int reset, ehd_fno_work* out);
 
 #ifdef __cplusplus
 }

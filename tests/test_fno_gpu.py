@@ -30,6 +30,67 @@ def test_gemm_matches_bf16_reference(mnk):
     assert np.all(np.abs(got - ref) <= 1e-5 * scale + 1e-6), np.max(np.abs(got - ref) / (scale + 1e-30))
 
 
+def _up4(v):
+    return (v + 3) // 4 * 4
+
+
+def _operand(layout, R, K, nb, rng):
+    """Flat host array of nb (R x K) matrices (R = m for A, n for B) in the
+    given layout; returns (flat, (r stride, k stride, batch stride), dense)."""
+    if layout == "k":        # k unit-stride, rows 16-byte aligned
+        sr, sk = _up4(K + 1), 1
+    elif layout == "r":      # m / n unit-stride
+        sr, sk = 1, _up4(R + 1)
+    else:                    # odd strides: element staging
+        sr, sk = 3, 3 * R + 1
+    sb = _up4((R - 1) * sr + (K - 1) * sk + 5)
+    flat = rng.standard_normal(nb * sb).astype(np.float32)
+    r, k = np.meshgrid(np.arange(R), np.arange(K), indexing="ij")
+    dense = np.stack([flat[b * sb + r * sr + k * sk] for b in range(nb)])
+    return flat, (sr, sk, sb), dense
+
+
+@pytest.mark.parametrize("la", ["k", "r", "x"])
+@pytest.mark.parametrize("lb", ["k", "r", "x"])
+@pytest.mark.parametrize("mnk", [(64, 128, 96), (36, 44, 28), (37, 45, 29), (8, 300, 4), (130, 24, 64)])
+def test_gemm_strided_layouts(la, lb, mnk):
+    """Every staging layout of A (k, m or neither unit-stride) and B (k, n or
+    neither), full and ragged tiles, several k-steps, batches, alpha, beta
+    and split-K slices, against FP64 products of the bf16-rounded operands."""
+    M, N, K = mnk
+    nb = 3
+    rng = np.random.default_rng(M * 7 + N * 3 + K + ord(la) * 11 + ord(lb))
+    A, (am, ak, a1), Ad = _operand(la, M, K, nb, rng)
+    Bt, (bn, bk, b1), Btd = _operand(lb, N, K, nb, rng)     # B stored as its transpose's layout: rows n
+    for ksplit in (0, 32):
+        ns = -(-K // ksplit) if ksplit else 1
+        cm, cn = N + 3, 1
+        c2 = M * cm + 1
+        c1 = ns * c2 + 7
+        C0 = rng.standard_normal(nb * c1).astype(np.float32)
+        alpha, beta = (1.0, 0.0) if ksplit else (-0.75, 0.5)
+        got = G.gemm_strided(M, N, K, A, (am, ak, a1), Bt, (bk, bn, b1), C0.copy(), (cm, cn, c1, c2), nb=nb,
+                             ksplit=ksplit, alpha=alpha, beta=beta)
+        for b in range(nb):
+            Ab = _bf16(Ad[b]).astype(np.float64)
+            Bb = _bf16(Btd[b]).astype(np.float64).T
+            for sl in range(ns):
+                k0, k1 = (sl * ksplit, min(K, (sl + 1) * ksplit)) if ksplit else (0, K)
+                m, n = np.meshgrid(np.arange(M), np.arange(N), indexing="ij")
+                idx = b * c1 + sl * c2 + m * cm + n * cn
+                ref = alpha * (Ab[:, k0:k1] @ Bb[k0:k1]) + beta * C0[idx].astype(np.float64)
+                scale = np.abs(Ab[:, k0:k1]) @ np.abs(Bb[k0:k1]) + abs(beta) * np.abs(C0[idx])
+                err = np.abs(got[idx] - ref)
+                assert np.all(err <= 1e-5 * scale + 1e-6), (b, sl, float(np.max(err / (scale + 1e-30))))
+        # elements outside C's tiles are untouched
+        mask = np.ones(C0.size, bool)
+        for b in range(nb):
+            for sl in range(ns):
+                m, n = np.meshgrid(np.arange(M), np.arange(N), indexing="ij")
+                mask[(b * c1 + sl * c2 + m * cm + n * cn).ravel()] = False
+        assert np.array_equal(got[mask], C0[mask])
+
+
 CFG = F.FNOConfig(X=16, Y=12, Z=8, width=8, modes=(4, 3, 3), layers=3, d_in=6, d_hidden=16, d_out=5)
 
 
@@ -76,8 +137,51 @@ def test_gradients_match_oracle():
     assert max(errs.values()) < 5e-2, errs
 
 
+def test_gradients_with_split_k_slices_match_oracle():
+    """A grid of more points than one split-K chunk of the weight-gradient
+    GEMMs (16x16x12 = 3072 > 2048), so those gradients are sums over several
+    slices per sample."""
+    cfg = F.FNOConfig(X=16, Y=16, Z=12, width=8, modes=(3, 4, 2), layers=2, d_in=6, d_hidden=16, d_out=5)
+    rng = np.random.default_rng(5)
+    p = F.init_params(cfg, seed=5)
+    for k in p:
+        if k in ("Pb", "Q1b", "Q2b") or k.startswith("b"):
+            p[k] = 0.1 * rng.standard_normal(p[k].shape)
+        if k.startswith("R"):
+            p[k] = p[k] * 20.0
+    a = rng.standard_normal((2, cfg.d_in, cfg.Z, cfg.Y, cfg.X))
+    t = rng.standard_normal((2, cfg.d_out, cfg.Z, cfg.Y, cfg.X))
+    net = G.FNO(cfg.X, cfg.Y, cfg.Z, width=cfg.width, modes=cfg.modes, layers=cfg.layers, d_in=cfg.d_in,
+                d_hidden=cfg.d_hidden, d_out=cfg.d_out, max_batch=2)
+    net.set_params(p)
+    out, cache = F.forward(cfg, p, a)
+    loss_ref, dout = F.mse(out, t)
+    gref = F.backward(cfg, p, cache, dout)
+    loss, g = net.gradients(a, t)
+    assert abs(loss / loss_ref - 1) < 1e-2, (loss, loss_ref)
+    gg = G.unflatten(net.cfg, g)
+    errs = {k: _rel(gg[k], gref[k]) for k in gref}
+    print("gradient rel-L2 per tensor (split-K):", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert max(errs.values()) < 5e-2, errs
+
+
 def test_adam_steps_reduce_loss():
     p, a, t, net = _setup(seed=2)
     losses = [net.train_step(a, t, 3e-3) for _ in range(30)]
     print("losses:", [f"{x:.4f}" for x in losses[::5]])
     assert losses[-1] < 0.9 * losses[0]
+
+
+def test_resident_batch_steps_match_host_batches():
+    """Training steps on the batch left on the device (no host transfer) give
+    bit for bit what the same steps with the host arrays give; a forward call
+    in between invalidates the resident batch."""
+    p, a, t, n1 = _setup(seed=3)
+    _, _, _, n2 = _setup(seed=3)
+    l1 = [n1.train_step(a, t, 2e-3) for _ in range(3)]
+    l2 = [n2.train_step(a, t, 2e-3)] + [n2.train_step(None, None, 2e-3) for _ in range(2)]
+    assert l1 == l2
+    assert np.array_equal(n1.get_params(), n2.get_params())
+    n2.forward(a)
+    with pytest.raises(Exception):
+        n2.train_step(None, None, 2e-3, batch=a.shape[0])
