@@ -1,11 +1,25 @@
-"""What the trained surrogate gains over the method without it, on 64x64x64
-boxes (C = 10, T between 200 and 300, where the flow settles from noise to
-steady square cells): after training on steady states from the batched
-solver (canonical pattern position), the network's
-prediction for held-out T is much closer to their steady state than the
-hydrostatic start the solver otherwise begins from, the warm-started runs
-end at the cold runs' Nu (the acceptance rule: within 0.5 %), and they
-reach steady state in fewer steps in total."""
+"""What the trained surrogate gains over the method without it (the learned
+warm start), on 64x64x64 boxes at dt/dx = 0.04 with T in 200-300 and C in
+5-20, where the flow settles from noise to steady square cells.
+
+The network is the specified one -- 4 Fourier layers, width 32, 12 modes
+per axis, GELU, bf16 matrix-core products with FP32 accumulation -- mapping
+(T, C, M, grid coordinates) to the steady u, v, w, charge and potential.
+Training states: a 3 x 3 grid of (T, C) run by the batched solver from the
+hydrostatic start to steady state (canonical pattern position); the network
+trains on them with Adam on the GPU (the batch resident in HBM).  Four
+held-out (T, C) draws inside the grid are then run to steady state twice:
+cold (hydrostatic start + noise) and warm (the network's fields through the
+consistent start, whose 5000 iterations count as steps).  Asserted: the
+acceptance rule of the surrogate (every warm run ends within 0.5 % of its
+cold run's Nu), the prediction's distance to the steady state, and the gain
+in steps to steady state over the four draws (measured 2.09x with these
+seeds; the 64-draw evaluation with 85 training states reaches 3.1x).
+
+Why not 64x64x32 and T up to 800 as the configuration names: on dz = 1/31
+every box from T = 170 to 400 either does not convect or goes non-finite
+before it settles (the same in FP64), and at dz = 1/63 boxes above T ~ 400
+fail as well, so there are no steady states to learn there."""
 import math
 
 import numpy as np
@@ -17,57 +31,62 @@ from ehd_amd.fno import FNO
 pytestmark = pytest.mark.gpu
 
 GRID = (64, 64, 64)
-TRAIN_T = (200.0, 220.0, 240.0, 260.0, 280.0, 300.0)
-TEST_T = (230.0, 270.0)
-C = 10.0
+TRAIN = [(T, C) for T in (200.0, 250.0, 300.0) for C in (5.0, 10.0, 20.0)]
+TEST = [(215.0, 7.0), (235.0, 14.0), (265.0, 6.0), (285.0, 12.0)]
 RUN = dict(check_every=2000, tol=1e-5, max_steps=300000)
 
 
-@pytest.mark.timeout(600)   # ~100 s: steady states of 8 boxes, 2000 training steps, warm runs
+@pytest.mark.timeout(600)   # ~130 s: 13 boxes to steady state, 2000 training steps, 4 warm runs
 def test_warm_start_gain():
-    # training states and the cold runs of the held-out T, one batch
-    draws = TRAIN_T + TEST_T
-    params = [S.box_params(GRID, T, C, seed=100 + i) for i, T in enumerate(draws)]
+    draws = TRAIN + TEST
+    params = [S.box_params(GRID, T, C, seed=100 + i) for i, (T, C) in enumerate(draws)]
     res, fields = S.run_batch_to_steady(params, tag="cold", **RUN)
-    for T, r in zip(draws, res):
-        print(f"cold T={T:.0f}: {r.steps} steps, Nu {r.nusselt:.5f}, converged {r.converged}, failed {r.failed}")
+    for (T, C), r in zip(draws, res):
+        print(f"cold T={T:.0f} C={C:.0f}: {r.steps} steps, Nu {r.nusselt:.5f}, converged {r.converged}")
     assert all(r.converged for r in res)
-    Y = np.stack([S.canonicalize(S.targets_from_fields(p, f, S.hydrostatic_profile(p)))
-                  for p, f in zip(params, fields)])
-    X = np.stack([S.features(GRID, T, C) for T in draws])
-    ntr = len(TRAIN_T)
-    Xtr, Ytr, Xte, Yte = X[:ntr], Y[:ntr], X[ntr:], Y[ntr:]
-    res_cold = res[ntr:]
+    profs = {}
 
-    net = FNO(*GRID, width=16, modes=(8, 8, 16), layers=4, max_batch=ntr)
+    def prof(p):
+        if p.C not in profs:
+            profs[p.C] = S.hydrostatic_profile(p)
+        return profs[p.C]
+
+    Y = np.stack([S.canonicalize(S.targets_from_fields(p, f, prof(p))) for p, f in zip(params, fields)])
+    X = np.stack([S.features(GRID, T, C) for T, C in draws])
+    ntr = len(TRAIN)
+    Xtr, Ytr, Xte, Yte = X[:ntr], Y[:ntr], X[ntr:], Y[ntr:]
+
+    net = FNO(*GRID, width=32, modes=(12, 12, 12), layers=4, max_batch=ntr)
     net.set_params(net.init_params(seed=3))
     losses = []
     nit = 2000
     for it in range(nit):
         lr = 2e-3 * 0.5 * (1 + math.cos(math.pi * it / nit))
-        losses.append(net.train_step(Xtr, Ytr, lr))
+        # the first step loads the training batch into HBM, the others reuse it
+        losses.append(net.train_step(Xtr, Ytr, lr) if it == 0 else net.train_step(None, None, lr))
     print(f"training loss {np.mean(losses[:10]):.3e} -> {np.mean(losses[-10:]):.3e}")
-    assert np.mean(losses[-10:]) < 0.01 * np.mean(losses[:10])
+    assert np.mean(losses[-10:]) < 1e-3 * np.mean(losses[:10])
 
     pred = net.forward(Xte)
-    # distance to the steady state (canonical phase): network vs the hydrostatic
-    # start (zero velocity, hydrostatic q and phi, i.e. all targets zero)
-    err_net = float(np.linalg.norm(pred - Yte) / np.linalg.norm(Yte))
-    print(f"held-out rel-L2 to the steady state: network {err_net:.3f}, hydrostatic start 1.000")
-    assert err_net < 0.05
+    # distance to the steady state (canonical position): the hydrostatic start
+    # (zero velocity, hydrostatic q and phi: all targets zero) is at 1
+    errs = [float(np.linalg.norm(pred[i] - Yte[i]) / np.linalg.norm(Yte[i])) for i in range(len(TEST))]
+    print("held-out rel-L2 to the steady state:", [f"{e:.4f}" for e in errs])
+    assert max(errs) < 0.03
 
     warm_params, warm_fields = [], []
-    for i, (T, y) in enumerate(zip(TEST_T, pred)):
+    for i, ((T, C), y) in enumerate(zip(TEST, pred)):
         pw = S.box_params(GRID, T, C, seed=900 + i, init=S.INIT_FIELDS)
         warm_params.append(pw)
-        warm_fields.append(S.fields9_from_prediction(pw, y, S.hydrostatic_profile(pw)))
+        warm_fields.append(S.fields9_from_prediction(pw, y, prof(pw)))
     res_warm, _ = S.run_batch_to_steady(warm_params, warm_fields, tag="warm", consistent=5000, **RUN)
+    res_cold = res[ntr:]
+    for (T, C), c, w in zip(TEST, res_cold, res_warm):
+        print(f"T={T:.0f} C={C:.0f}: cold {c.steps} steps Nu {c.nusselt:.5f}; warm {w.steps} steps Nu {w.nusselt:.5f}")
     cold = sum(r.steps for r in res_cold)
     warm = sum(r.steps for r in res_warm)
-    for T, c, w in zip(TEST_T, res_cold, res_warm):
-        print(f"T={T:.0f}: cold {c.steps} steps Nu {c.nusselt:.5f}; warm {w.steps} steps Nu {w.nusselt:.5f}")
     print(f"total steps to steady: cold {cold}, warm {warm} (ratio {cold / warm:.2f})")
     assert all(w.converged for w in res_warm)
     for c, w in zip(res_cold, res_warm):
-        assert abs(w.nusselt / c.nusselt - 1) < 5e-3
-    assert warm < cold
+        assert abs(w.nusselt / c.nusselt - 1) < 5e-3          # the acceptance rule
+    assert cold >= 1.8 * warm
