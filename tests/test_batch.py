@@ -101,3 +101,27 @@ def test_batch_steady_detector():
             assert abs(r.nusselt - 1.0) < 1e-3
         d = bt.diagnostics()
         assert all(x.step == max(r.steps for r in res) for x in d)
+
+
+@pytest.mark.parametrize("flags", [E.BATCH_GRAPH | E.BATCH_TUNE, E.BATCH_GRAPH | E.BATCH_CHAINS])
+def test_batch_with_a_256_wide_box_matches_single_boxes(flags):
+    """The bench box's plane size (256 x 256: one LBM tile per x-row, the
+    compile-time FFT plans of 256) in a batch beside a box of another plane
+    size: every box as alone, bit for bit, with graph replay, with the tuner
+    and with chains."""
+    params = [E.make_params(256, 256, 8, T=400.0, C_=10.0, lam=0.25, precision=E.FP32S,
+                            init_mode=E.INIT_HYDROSTATIC, seed=21, amp=1e-3),
+              E.make_params(64, 64, 20, T=300.0, C_=5.0, lam=0.25, precision=E.FP32S, init_mode=E.INIT_HYDROSTATIC,
+                            seed=22, amp=1e-3)]
+    nsteps = (1, 9, 14)
+    with E.Batch(params, flags=flags) as bt:
+        bt.init()
+        for n in nsteps:
+            bt.step(n)
+        got = [_state(b) for b in bt.boxes]
+    for i, p in enumerate(params):
+        with E.Box(p) as b:
+            b.init()
+            for n in nsteps:
+                b.step(n)
+            _assert_same_state(got[i], _state(b), f"box {i} ({p.nx}x{p.ny}x{p.nz})")
