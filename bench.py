@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--precision", choices=("fp32s", "fp64"), default="fp32s")
     ap.add_argument("--T", type=float, default=400.0, help="electric Rayleigh number of the box")
     ap.add_argument("--seed", type=int, default=2, help="seed of the 1e-3 velocity noise")
-    ap.add_argument("--cpu-steps", type=int, default=5,
+    ap.add_argument("--cpu-steps", type=int, default=8,
                     help="oracle steps per timed run of the CPU sample, three runs (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle OpenMP threads (0: OMP_NUM_THREADS, else the cores this process may use)")
