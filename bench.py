@@ -278,8 +278,8 @@ def roofline(E, args, prec, prof, workload):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the coupled step is "
-              "single-GPU in this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the coupled step, the ensemble "
+              "and the surrogate run on a single GPU", file=sys.stderr)
         sys.exit(2)
     if args.workload == "ensemble":
         run_ensemble(args)
