@@ -115,3 +115,35 @@ def test_hydrostatic_current_matches_oracle(C):
     got = E.hydrostatic_current(E.make_params(nx, ny, nz, T=200.0, C_=C, lam=0.25))
     assert ref > 0
     assert abs(got - ref) <= 1e-13 * ref, (got, ref)
+
+
+def test_strided_gemm_rejects_bad_layouts_on_host():
+    """ehd_fno_gemm_strided checks its arguments before any device work:
+    strides that reach outside the host arrays, negative strides and empty
+    shapes are refused with EHD_EINVAL and a message."""
+    import ctypes as C
+    import numpy as np
+    from ehd_amd import fno as G
+
+    L = G._lib()
+    A = np.zeros(64, np.float32)
+    B = np.zeros(64, np.float32)
+    Cm = np.zeros(64, np.float32)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    arr = lambda v: (C.c_longlong * len(v))(*v)
+    # 4 x 4 x 4 fits 16-element operands with unit strides
+    ok = (4, 1, 16)
+    # A reaches element 4*5 + 3 = 23 > 15 of a 16-element array
+    rc = L.ehd_fno_gemm_strided(4, 4, 4, 1, 0, fp(A), arr((5, 1, 0)), 16, fp(B), arr(ok), 64, fp(Cm),
+                                arr((4, 1, 16, 0)), 64, 1.0, 0.0)
+    assert rc == -1 and b"outside" in L.ehd_last_error()
+    rc = L.ehd_fno_gemm_strided(4, 4, 4, 1, 0, fp(A), arr((4, -1, 0)), 64, fp(B), arr(ok), 64, fp(Cm),
+                                arr((4, 1, 16, 0)), 64, 1.0, 0.0)
+    assert rc == -1 and b"negative" in L.ehd_last_error()
+    rc = L.ehd_fno_gemm_strided(0, 4, 4, 1, 0, fp(A), arr(ok), 64, fp(B), arr(ok), 64, fp(Cm),
+                                arr((4, 1, 16, 0)), 64, 1.0, 0.0)
+    assert rc == -1
+    # split-K slices: 8 slices of k = 32 / 4 need C slices up to 7 * 16 + 15 = 127 > 63
+    rc = L.ehd_fno_gemm_strided(4, 4, 32, 1, 4, fp(A), arr((0, 1, 0)), 64, fp(B), arr((1, 0, 0)), 64, fp(Cm),
+                                arr((4, 1, 0, 16)), 64, 1.0, 0.0)
+    assert rc == -1 and b"outside" in L.ehd_last_error()
