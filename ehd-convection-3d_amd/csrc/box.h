@@ -49,6 +49,7 @@ struct ehd_box {
     void* lbm_items[2] = {nullptr, nullptr};   // FP32S: one-item kernel tables per buffer parity
     unsigned lbm_chunk = 0;              // FP32S LBM XCD tile order (lbm_xcd_chunk)
     int tune_lpb = 0, tune_ylines = 0;   // Poisson lines per block (0: largest that fits)
+    bool debug_sync = false;   // synchronise and check after every launch (ehd_set_debug_sync)
     long long step = 0;
     double t = 0.0;
     double current_hs = 0.0;

@@ -180,6 +180,12 @@ int run_lbm(ehd_box* b, bool macro, void* const* target_items)
                                    (unsigned)((b->S + 255) / 256), 0, b->lbm_chunk, b->direct, b->ebuf, macro, b->st,
                                    target_items != nullptr);
     if (e != hipSuccess) return set_error(EHD_EHIP, "LBM launch failed: %s", hipGetErrorString(e));
+    if (b->debug_sync) {
+        e = hipStreamSynchronize(b->st);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess)
+            return set_error(EHD_EHIP, "LBM launch of step %lld failed: %s", b->step, hipGetErrorString(e));
+    }
     if (prof) {
         HIPCHK(hipEventRecord(e1, b->st));
         b->lbm_pairs.emplace_back(e0, e1);
@@ -203,6 +209,7 @@ PoissonGroup poisson_group_of(const ehd_box* b)
     g.twx = b->twx;
     g.twy = b->twy;
     g.jobs = b->pjob;
+    g.debug_sync = b->debug_sync ? 1 : 0;
     return g;
 }
 
@@ -217,7 +224,12 @@ int run_poisson(ehd_box* b)
         HIPCHK(hipEventRecord(e0, b->st));
     }
     hipError_t e = launch_poisson_group(b->prec, poisson_group_of(b), b->st);
-    if (e != hipSuccess) return set_error(EHD_EHIP, "Poisson launch failed: %s", hipGetErrorString(e));
+    if (e != hipSuccess) {
+        if (b->debug_sync)
+            return set_error(EHD_EHIP, "Poisson solve of step %lld failed in the %s: %s", b->step,
+                             poisson_failed_pass(), hipGetErrorString(e));
+        return set_error(EHD_EHIP, "Poisson launch failed: %s", hipGetErrorString(e));
+    }
     if (prof) {
         HIPCHK(hipEventRecord(e1, b->st));
         b->poi_pairs.emplace_back(e0, e1);
@@ -1027,6 +1039,13 @@ int ehd_run_to_steady(ehd_box* b, int check_every, double tol, long long max_ste
     out->steps = taken;
     out->nusselt = dg.nusselt;
     out->current = dg.current;
+    return EHD_OK;
+}
+
+int ehd_set_debug_sync(ehd_box* b, int on)
+{
+    if (!b) return set_error(EHD_EINVAL, "null box");
+    b->debug_sync = on != 0;
     return EHD_OK;
 }
 

@@ -125,7 +125,10 @@ struct PoissonGroup {
     const void* twx;             // twiddles of nx and ny, path type
     const void* twy;
     const PoissonJob* jobs;      // device array of njobs
+    int debug_sync;              // synchronise the stream and check after every pass
 };
+// name of the Poisson pass whose synchronised check failed last (debug_sync)
+const char* poisson_failed_pass();
 struct PoissonShape {
     int max_x_lines, max_y_lines, zs_partition;
 };

@@ -519,6 +519,8 @@ __global__ void __launch_bounds__(256) ffty_kernel(const PoissonJob* __restrict_
 // LDS: 6 P MB doubles.
 constexpr int ZS_MAXM = 16;
 
+thread_local const char* g_failed_pass = "";
+
 template <typename T, int P, int MAXM, int NT = 256>
 __global__ void __launch_bounds__(NT) zsolve_kernel(const PoissonJob* __restrict__ jobs, int nmodes, int ny,
                                                       int split_mean, double inv_nxy)
@@ -903,8 +905,19 @@ hipError_t poisson_group(const PoissonGroup& g, hipStream_t st)
     const cplx<T>* twx = (const cplx<T>*)g.twx;
     const cplx<T>* twy = (const cplx<T>*)g.twy;
 
+    // debug_sync: each pass synchronised and checked on its own
+    auto check = [&](const char* pass) -> hipError_t {
+        if (!g.debug_sync) return hipSuccess;
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) g_failed_pass = pass;
+        return e;
+    };
+    hipError_t ec;
     fftx_fwd_kernel<T, NX><<<gx, 256, shx, st>>>(g.jobs, twx, plx, nx, ny, lpb);
+    if ((ec = check("forward x transform")) != hipSuccess) return ec;
     ffty_kernel<T, NY><<<gy, 256, shy, st>>>(g.jobs, twy, ply, ny, nxh, G, -1);
+    if ((ec = check("forward y transform")) != hipSuccess) return ec;
 
     const int nmodes = nxh * ny;
     const int Pz = zsolve_partition(nmax, cs);
@@ -931,10 +944,16 @@ hipError_t poisson_group(const PoissonGroup& g, hipStream_t st)
         ez = hipSuccess;
     }
     if (ez != hipSuccess) return ez;
+    if ((ec = check("z solve")) != hipSuccess) return ec;
 
     ffty_kernel<T, NY><<<gy, 256, shy, st>>>(g.jobs, twy, ply, ny, nxh, G, +1);
+    if ((ec = check("inverse y transform")) != hipSuccess) return ec;
     fftx_inv_kernel<T, NX><<<gx, 256, shx, st>>>(g.jobs, twx, plx, nx, ny, lpb, sm ? 0 : 1);
-    if (!sm) phi_walls_kernel<<<dim3((nx * ny + 255) / 256, g.njobs), 256, 0, st>>>(g.jobs, nx * ny);
+    if ((ec = check("inverse x transform")) != hipSuccess) return ec;
+    if (!sm) {
+        phi_walls_kernel<<<dim3((nx * ny + 255) / 256, g.njobs), 256, 0, st>>>(g.jobs, nx * ny);
+        if ((ec = check("wall potential")) != hipSuccess) return ec;
+    }
     return hipGetLastError();
 }
 
@@ -1028,5 +1047,7 @@ hipError_t launch_poisson_group(Prec prec, const PoissonGroup& g, hipStream_t st
     if (prec == Prec::F64) return poisson_t<double>(g, st);
     return poisson_t<float>(g, st);
 }
+
+const char* poisson_failed_pass() { return g_failed_pass; }
 
 }  // namespace ehd

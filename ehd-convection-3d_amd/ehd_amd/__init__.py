@@ -84,7 +84,7 @@ EXPORTS = (
     "ehd_params_default", "ehd_derive", "ehd_compute_parameters", "ehd_create", "ehd_destroy",
     "ehd_init", "ehd_step", "ehd_lbm_step", "ehd_poisson", "ehd_read_fields", "ehd_write_phi",
     "ehd_write_charge", "ehd_read_populations", "ehd_write_populations", "ehd_diagnostics",
-    "ehd_set_nusselt_reference", "ehd_profile_enable", "ehd_profile_read", "ehd_profile_reset",
+    "ehd_set_nusselt_reference", "ehd_profile_enable", "ehd_profile_read", "ehd_profile_reset", "ehd_set_debug_sync",
     "ehd_sync", "ehd_last_error", "ehd_abi_version", "ehd_save_state", "ehd_load_state",
     "ehd_hydrostatic_current", "ehd_run_to_steady", "ehd_hydrostatic_reference",
     "ehd_init_consistent", "ehd_batch_init_consistent",
@@ -127,6 +127,7 @@ def lib() -> C.CDLL:
     L.ehd_diagnostics.argtypes = [vp, C.POINTER(Diag)]
     L.ehd_set_nusselt_reference.argtypes = [vp, C.c_double]
     L.ehd_profile_enable.argtypes = [vp, C.c_int]
+    L.ehd_set_debug_sync.argtypes = [vp, C.c_int]
     L.ehd_profile_read.argtypes = [vp, C.POINTER(Profile)]
     L.ehd_profile_reset.argtypes = [vp]
     L.ehd_sync.argtypes = [vp]
@@ -317,6 +318,10 @@ class Box:
 
     def set_nusselt_reference(self, current_hs: float):
         _check(lib().ehd_set_nusselt_reference(self._h, float(current_hs)))
+
+    def set_debug_sync(self, on: bool = True):
+        """Synchronise and check after every launch (faults named by phase and step)."""
+        _check(lib().ehd_set_debug_sync(self._h, int(on)))
 
     def profile_enable(self, on: bool = True, every: int = 1):
         """Events around the LBM and Poisson launches of every ``every``-th step."""

@@ -222,6 +222,15 @@ int ehd_profile_enable(ehd_box* b, int on);
 int ehd_profile_read(ehd_box* b, ehd_profile* out);
 int ehd_profile_reset(ehd_box* b);
 
+/* Debug mode (on != 0): every kernel launch of the box's steps and Poisson
+ * solves is followed by a synchronisation of the box's stream and an error
+ * check, so that an asynchronous fault is reported by the launch that caused
+ * it, named in ehd_last_error() (phase and step).  The reference checks
+ * launches only with cudaGetLastError and no synchronisation (LBM.cu:36-57),
+ * so its faults surface later.  Off by default; the results are the same
+ * either way, only slower. */
+int ehd_set_debug_sync(ehd_box* b, int on);
+
 int ehd_sync(ehd_box* b);
 const char* ehd_last_error(void);
 

@@ -67,3 +67,22 @@ def test_batch_watchdog_flags_one_box_and_steps_the_others():
         assert res[1].failed == 1 and res[1].converged == 0 and res[1].steps == 0
         for i in (0, 2):
             assert res[i].failed == 0 and res[i].converged == 1 and abs(res[i].nusselt - 1.0) < 1e-3
+
+
+def test_debug_sync_steps_match_and_report_by_phase():
+    """Debug mode (a synchronisation and an error check after every launch)
+    gives the same state as the normal step; it only moves where a fault is
+    reported."""
+    out = []
+    for dbg in (False, True):
+        p = E.make_params(32, 32, 16, T=300.0, C_=10.0, lam=0.25, precision=E.FP32S, init_mode=E.INIT_HYDROSTATIC,
+                          seed=4, amp=1e-3)
+        with E.Box(p) as b:
+            b.set_debug_sync(dbg)
+            b.init()
+            b.step(7)
+            out.append((b.read_fields(), b.read_populations()))
+    for k in out[0][0]:
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    for x, y in zip(out[0][1], out[1][1]):
+        assert np.array_equal(x, y)
