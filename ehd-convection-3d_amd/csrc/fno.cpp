@@ -71,6 +71,9 @@ struct ehd_fno {
     int ksplit = 0, nsplit = 1;   // K chunk of the weight-gradient GEMMs over the grid points
     ehd_fno_work work{};          // GEMM work since the last reset
     int resident = 0;             // samples of the last inputs/targets copied to the device
+    float* ds_in = nullptr;       // resident training set (ehd_fno_set_dataset)
+    float* ds_tgt = nullptr;
+    int ds_n = 0;
 };
 
 namespace {
@@ -139,8 +142,18 @@ int calloc_c(ehd_fno* f, Cbuf* b, long long n)
     return rc ? rc : falloc(f, &b->i, n);
 }
 
+void free_dataset(ehd_fno* f)
+{
+    if (f->st) (void)hipStreamSynchronize(f->st);
+    if (f->ds_in) (void)hipFree(f->ds_in);
+    if (f->ds_tgt) (void)hipFree(f->ds_tgt);
+    f->ds_in = f->ds_tgt = nullptr;
+    f->ds_n = 0;
+}
+
 void free_fno(ehd_fno* f)
 {
+    free_dataset(f);
     if (f->st) (void)hipStreamSynchronize(f->st);
     for (void* q : f->allocs) (void)hipFree(q);
     f->allocs.clear();
@@ -666,6 +679,46 @@ int ehd_fno_train_step(ehd_fno* f, int batch, const float* in, const float* targ
     FCHK(adam(f->p, f->g, f->m1, f->m2, f->nparam, lr, 0.9f, 0.999f, 1e-8f, f->adam_t, f->st));
     HIPCHK(hipStreamSynchronize(f->st));
     return EHD_OK;
+}
+
+int ehd_fno_set_dataset(ehd_fno* f, int n, const float* in, const float* target)
+{
+    if (!f || n < 1 || !in || !target) return set_error(EHD_EINVAL, "bad dataset arguments");
+    free_dataset(f);
+    const size_t nin = (size_t)n * f->c.d_in * f->ZP, nout = (size_t)n * f->c.d_out * f->ZP;
+    if (hipMalloc((void**)&f->ds_in, nin * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&f->ds_tgt, nout * sizeof(float)) != hipSuccess) {
+        free_dataset(f);
+        return set_error(EHD_ENOMEM, "dataset of %d samples (%.1f GB) does not fit", n,
+                         (double)(nin + nout) * 4e-9);
+    }
+    HIPCHK(hipMemcpyAsync(f->ds_in, in, nin * sizeof(float), hipMemcpyHostToDevice, f->st));
+    HIPCHK(hipMemcpyAsync(f->ds_tgt, target, nout * sizeof(float), hipMemcpyHostToDevice, f->st));
+    HIPCHK(hipStreamSynchronize(f->st));
+    f->ds_n = n;
+    return EHD_OK;
+}
+
+int ehd_fno_train_step_indices(ehd_fno* f, int batch, const int* idx, float lr, double* loss)
+{
+    if (!f || !idx) return set_error(EHD_EINVAL, "null argument");
+    int rc = check_batch(f, batch);
+    if (rc) return rc;
+    if (f->ds_n == 0) return set_error(EHD_ESTATE, "no resident dataset (ehd_fno_set_dataset)");
+    const size_t sin = (size_t)f->c.d_in * f->ZP, sout = (size_t)f->c.d_out * f->ZP;
+    for (int i = 0; i < batch; ++i) {
+        if (idx[i] < 0 || idx[i] >= f->ds_n)
+            return set_error(EHD_EINVAL, "sample index %d outside 0..%d", idx[i], f->ds_n - 1);
+    }
+    // gather the batch on the device; the step then runs on it as on a resident batch
+    for (int i = 0; i < batch; ++i) {
+        HIPCHK(hipMemcpyAsync(f->in + i * sin, f->ds_in + (size_t)idx[i] * sin, sin * sizeof(float),
+                              hipMemcpyDeviceToDevice, f->st));
+        HIPCHK(hipMemcpyAsync(f->tgt + i * sout, f->ds_tgt + (size_t)idx[i] * sout, sout * sizeof(float),
+                              hipMemcpyDeviceToDevice, f->st));
+    }
+    f->resident = batch;
+    return ehd_fno_train_step(f, batch, nullptr, nullptr, lr, loss);
 }
 
 int ehd_fno_gemm_work(ehd_fno* f, int reset, ehd_fno_work* out)

@@ -22,7 +22,7 @@ class Work(C.Structure):
 
 EXPORTS = ("ehd_fno_create", "ehd_fno_destroy", "ehd_fno_num_params", "ehd_fno_set_params", "ehd_fno_get_params",
            "ehd_fno_forward", "ehd_fno_gradients", "ehd_fno_train_step", "ehd_fno_gemm", "ehd_fno_gemm_strided",
-           "ehd_fno_gemm_work")
+           "ehd_fno_gemm_work", "ehd_fno_set_dataset", "ehd_fno_train_step_indices")
 
 _bound = False
 
@@ -49,8 +49,12 @@ def _lib():
         L.ehd_fno_gemm_strided.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fp, llp, C.c_longlong, fp,
                                            llp, C.c_longlong, fp, llp, C.c_longlong, C.c_float, C.c_float]
         L.ehd_fno_gemm_work.argtypes = [vp, C.c_int, C.POINTER(Work)]
+        L.ehd_fno_set_dataset.argtypes = [vp, C.c_int, fp, fp]
+        L.ehd_fno_train_step_indices.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_float,
+                                                 C.POINTER(C.c_double)]
         for n in ("ehd_fno_set_params", "ehd_fno_get_params", "ehd_fno_forward", "ehd_fno_gradients",
-                  "ehd_fno_train_step", "ehd_fno_gemm", "ehd_fno_gemm_strided", "ehd_fno_gemm_work"):
+                  "ehd_fno_train_step", "ehd_fno_gemm", "ehd_fno_gemm_strided", "ehd_fno_gemm_work",
+                  "ehd_fno_set_dataset", "ehd_fno_train_step_indices"):
             getattr(L, n).restype = C.c_int
         _bound = True
     return L
@@ -180,6 +184,22 @@ class FNO:
         w = Work()
         _check(_lib().ehd_fno_gemm_work(self._h, int(reset), C.byref(w)))
         return w
+
+    def set_dataset(self, a, target):
+        """Copy a training set ([n][d_in][Z][Y][X], [n][d_out][Z][Y][X]) into HBM once."""
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        assert a.shape[0] == t.shape[0]
+        _check(_lib().ehd_fno_set_dataset(self._h, a.shape[0], _fp(a), _fp(t)))
+
+    def train_step_indices(self, idx, lr: float) -> float:
+        """One Adam step on the resident samples idx (gathered on the device)."""
+        ix = np.ascontiguousarray(idx, dtype=np.int32)
+        loss = C.c_double()
+        _check(_lib().ehd_fno_train_step_indices(self._h, ix.size, ix.ctypes.data_as(C.POINTER(C.c_int)),
+                                                 float(lr), C.byref(loss)))
+        self._resident = ix.size
+        return loss.value
 
     def train_step(self, a, target, lr: float, batch: int = 0) -> float:
         """One Adam step on (a, target); a and target None: on the batch the

@@ -383,6 +383,7 @@ def main(argv=None):
 
         # 2. training on the GPU (the library's forward/backward kernels, Adam)
         net.set_params(net.init_params(seed=args.seed))
+        net.set_dataset(X, Y)        # resident in HBM: mini-batches are gathered on the device
         ntr = len(X)
         bs = 8
         steps_per_epoch = max(1, ntr // bs)
@@ -396,7 +397,7 @@ def main(argv=None):
             for j in range(steps_per_epoch):
                 idx = order[j * bs:(j + 1) * bs]
                 lr = args.lr * 0.5 * (1 + math.cos(math.pi * k / total))
-                el.append(net.train_step(X[idx], Y[idx], lr))
+                el.append(net.train_step_indices(idx, lr))
                 k += 1
             losses.append(float(np.mean(el)))
             if ep % 5 == 0 or ep == args.epochs - 1:

@@ -60,6 +60,15 @@ int ehd_fno_gradients(ehd_fno* f, int batch, const float* in, const float* targe
  * as in ehd_fno_gradients */
 int ehd_fno_train_step(ehd_fno* f, int batch, const float* in, const float* target, float lr, double* loss);
 
+/* A training set resident in HBM: n samples of inputs [n][d_in][Z][Y][X] and
+ * targets [n][d_out][Z][Y][X], copied once (replacing any earlier set; the
+ * library allocates it, EHD_ENOMEM if it does not fit). */
+int ehd_fno_set_dataset(ehd_fno* f, int n, const float* in, const float* target);
+
+/* One Adam step (as ehd_fno_train_step) on the samples idx[0..batch) of the
+ * resident set, gathered on the device: no host transfer per step. */
+int ehd_fno_train_step_indices(ehd_fno* f, int batch, const int* idx, float lr, double* loss);
+
 /* C = A B for row-major host matrices (M x K, K x N) through the bf16
  * matrix-core GEMM the network uses (checks of its operand layout) */
 int ehd_fno_gemm(int M, int N, int K, const float* A, const float* B, float* C);

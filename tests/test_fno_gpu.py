@@ -185,3 +185,28 @@ def test_resident_batch_steps_match_host_batches():
     n2.forward(a)
     with pytest.raises(Exception):
         n2.train_step(None, None, 2e-3, batch=a.shape[0])
+
+
+def test_resident_dataset_steps_match_host_batches():
+    """Steps on mini-batches gathered on the device from a resident training
+    set give bit for bit what the same steps on host copies of those samples
+    give; out-of-range indices and a batch above max_batch are refused."""
+    rng = np.random.default_rng(11)
+    _, _, _, n1 = _setup(seed=5)
+    _, _, _, n2 = _setup(seed=5)
+    ds_a = rng.standard_normal((7, CFG.d_in, CFG.Z, CFG.Y, CFG.X)).astype(np.float32)
+    ds_t = rng.standard_normal((7, CFG.d_out, CFG.Z, CFG.Y, CFG.X)).astype(np.float32)
+    n2.set_dataset(ds_a, ds_t)
+    for idx in ([6, 0, 3], [2, 2, 5, 1], [4]):
+        l1 = n1.train_step(ds_a[idx], ds_t[idx], 2e-3)
+        l2 = n2.train_step_indices(idx, 2e-3)
+        assert l1 == l2
+    assert np.array_equal(n1.get_params(), n2.get_params())
+    with pytest.raises(Exception):
+        n2.train_step_indices([0, 7], 2e-3)
+    with pytest.raises(Exception):
+        n2.train_step_indices([0, 1, 2, 3, 4], 2e-3)
+    # a new set replaces the old one
+    n2.set_dataset(ds_a[:2], ds_t[:2])
+    with pytest.raises(Exception):
+        n2.train_step_indices([2], 2e-3)
