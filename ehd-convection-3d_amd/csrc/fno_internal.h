@@ -27,6 +27,9 @@ hipError_t gemm_bf16(const GemmArgs& g, hipStream_t st);
 // (16-byte loads), 2 element loads
 int gemm_layout_a(const GemmArgs& g);
 int gemm_layout_b(const GemmArgs& g);
+// result layout: 0 element stores, 1 n unit-stride, 2 m unit-stride (16-byte
+// stores through LDS)
+int gemm_layout_c(const GemmArgs& g);
 // dst[i] = beta*dst[i] + sum_j src[j*n + i], j < nslices; part holds
 // sum_slices_part_floats(n) floats of workspace
 long long sum_slices_part_floats(long long n);

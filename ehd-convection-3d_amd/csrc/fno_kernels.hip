@@ -48,6 +48,16 @@ constexpr int L_K = 0;       // k unit-stride: 16-byte loads along k
 constexpr int L_MN = 1;      // m (A) or n (B) unit-stride: 16-byte loads along m / n
 constexpr int L_ANY = 2;     // element loads
 
+// result layouts: with a unit-stride axis the C tile goes through LDS and
+// leaves as 16-byte stores along that axis (a wave writes four 256-byte
+// rows); otherwise each lane stores its MFMA elements (64-byte pieces)
+constexpr int C_ANY = 0;
+constexpr int C_N = 1;       // n unit-stride
+constexpr int C_M = 2;       // m unit-stride
+constexpr int CP = TN + 4;   // C tile pitch in LDS (floats)
+constexpr int kOperandLds = 2 * TM * LP * 2 + 2 * TN * LP * 2;
+constexpr int kGemmLds = kOperandLds > TM * CP * 4 ? kOperandLds : TM * CP * 4;
+
 // One operand tile (64 rows of m or n, 32 of k) in flight: 8 floats per thread.
 struct Stage {
     float v[8];
@@ -122,11 +132,13 @@ __device__ __forceinline__ void store_tile(const Stage& st, unsigned short (*T)[
     }
 }
 
-template <int AL, int BL>
+template <int AL, int BL, int CL>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
 {
-    __shared__ __align__(16) unsigned short As[2][TM][LP];
-    __shared__ __align__(16) unsigned short Bs[2][TN][LP];
+    // operand buffers during the k-steps, then the C tile
+    __shared__ __align__(16) unsigned char lds[kGemmLds];
+    unsigned short(*As)[TM][LP] = reinterpret_cast<unsigned short(*)[TM][LP]>(lds);
+    unsigned short(*Bs)[TN][LP] = reinterpret_cast<unsigned short(*)[TN][LP]>(lds + 2 * TM * LP * 2);
     const int bz = blockIdx.z;
     const int i1 = bz / g.nb2, i2 = bz - i1 * g.nb2;
     const float* __restrict__ A = g.A + i1 * g.a1 + i2 * g.a2;
@@ -178,19 +190,59 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
     }
     // D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4
     const int lj = lane & 15, li4 = (lane >> 4) * 4;
+    if constexpr (CL == C_ANY) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+            for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int gm = m0 + wm + a * 16 + li4 + e, gn = n0 + wn + b * 16 + lj;
-                if (gm < g.M && gn < g.N) {
-                    float* c = C + (long long)gm * g.cm + (long long)gn * g.cn;
-                    const float v = g.alpha * acc[a][b][e];
-                    *c = (g.beta != 0.f) ? v + g.beta * *c : v;
+                for (int e = 0; e < 4; ++e) {
+                    const int gm = m0 + wm + a * 16 + li4 + e, gn = n0 + wn + b * 16 + lj;
+                    if (gm < g.M && gn < g.N) {
+                        float* c = C + (long long)gm * g.cm + (long long)gn * g.cn;
+                        const float v = g.alpha * acc[a][b][e];
+                        *c = (g.beta != 0.f) ? v + g.beta * *c : v;
+                    }
                 }
+    } else {
+        // the last k-step's barrier has retired every LDS read of the operands
+        float(*Cs)[CP] = reinterpret_cast<float(*)[CP]>(lds);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int i = wm + a * 16 + li4 + e, j = wn + b * 16 + lj;
+                    if constexpr (CL == C_N) Cs[i][j] = acc[a][b][e];
+                    else Cs[j][i] = acc[a][b][e];
+                }
+        __syncthreads();
+        // 64 rows (m for C_N, n for C_M) of 16 quads along the unit-stride axis
+        const int r0 = CL == C_N ? m0 : n0, c0 = CL == C_N ? n0 : m0;
+        const int R = CL == C_N ? g.M : g.N, Q = CL == C_N ? g.N : g.M;
+        const long long sr = CL == C_N ? g.cm : g.cn;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int q = tid + 256 * h, r = q >> 4, c4 = 4 * (q & 15);
+            if (r0 + r < R && c0 + c4 < Q) {   // Q is a multiple of 4: the quad lies inside
+                float* c = C + (long long)(r0 + r) * sr + (c0 + c4);
+                float4 v = *(const float4*)&Cs[r][c4];
+                v.x = g.alpha * v.x;
+                v.y = g.alpha * v.y;
+                v.z = g.alpha * v.z;
+                v.w = g.alpha * v.w;
+                if (g.beta != 0.f) {
+                    const float4 o = *(const float4*)c;
+                    v.x = v.x + g.beta * o.x;
+                    v.y = v.y + g.beta * o.y;
+                    v.z = v.z + g.beta * o.z;
+                    v.w = v.w + g.beta * o.w;
+                }
+                *(float4*)c = v;
             }
+        }
+    }
 }
 
 // Sums of many slices and of channel rows, deterministic (fixed order, no
@@ -406,22 +458,43 @@ int gemm_layout_b(const GemmArgs& g)
     return L_ANY;
 }
 
+// Layout of C: 16-byte stores need the unit-stride axis in whole quads
+// (extent a multiple of 4) and every other offset a multiple of 4 floats
+// from a 16-byte aligned base.
+int gemm_layout_c(const GemmArgs& g)
+{
+    const bool base = aligned16(g.C) && mul4(g.c1) && mul4(g.c2);
+    if (base && g.cn == 1 && mul4(g.cm) && mul4(g.N)) return C_N;
+    if (base && g.cm == 1 && mul4(g.cn) && mul4(g.M)) return C_M;
+    return C_ANY;
+}
+
+namespace {
+template <int AL, int BL>
+void launch_gemm(int cl, dim3 grid, const GemmArgs& g, hipStream_t st)
+{
+    if (cl == C_N) gemm_bf16_kernel<AL, BL, C_N><<<grid, 256, 0, st>>>(g);
+    else if (cl == C_M) gemm_bf16_kernel<AL, BL, C_M><<<grid, 256, 0, st>>>(g);
+    else gemm_bf16_kernel<AL, BL, C_ANY><<<grid, 256, 0, st>>>(g);
+}
+}  // namespace
+
 hipError_t gemm_bf16(const GemmArgs& g, hipStream_t st)
 {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.nb1 <= 0 || g.nb2 <= 0) return hipErrorInvalidValue;
     dim3 grid((g.M + TM - 1) / TM, (g.N + TN - 1) / TN, g.nb1 * g.nb2);
     if (grid.y > 65535 || grid.z > 65535) return hipErrorInvalidValue;
-    const int al = gemm_layout_a(g), bl = gemm_layout_b(g);
+    const int al = gemm_layout_a(g), bl = gemm_layout_b(g), cl = gemm_layout_c(g);
     switch (al * 3 + bl) {
-    case 0: gemm_bf16_kernel<L_K, L_K><<<grid, 256, 0, st>>>(g); break;
-    case 1: gemm_bf16_kernel<L_K, L_MN><<<grid, 256, 0, st>>>(g); break;
-    case 2: gemm_bf16_kernel<L_K, L_ANY><<<grid, 256, 0, st>>>(g); break;
-    case 3: gemm_bf16_kernel<L_MN, L_K><<<grid, 256, 0, st>>>(g); break;
-    case 4: gemm_bf16_kernel<L_MN, L_MN><<<grid, 256, 0, st>>>(g); break;
-    case 5: gemm_bf16_kernel<L_MN, L_ANY><<<grid, 256, 0, st>>>(g); break;
-    case 6: gemm_bf16_kernel<L_ANY, L_K><<<grid, 256, 0, st>>>(g); break;
-    case 7: gemm_bf16_kernel<L_ANY, L_MN><<<grid, 256, 0, st>>>(g); break;
-    default: gemm_bf16_kernel<L_ANY, L_ANY><<<grid, 256, 0, st>>>(g); break;
+    case 0: launch_gemm<L_K, L_K>(cl, grid, g, st); break;
+    case 1: launch_gemm<L_K, L_MN>(cl, grid, g, st); break;
+    case 2: launch_gemm<L_K, L_ANY>(cl, grid, g, st); break;
+    case 3: launch_gemm<L_MN, L_K>(cl, grid, g, st); break;
+    case 4: launch_gemm<L_MN, L_MN>(cl, grid, g, st); break;
+    case 5: launch_gemm<L_MN, L_ANY>(cl, grid, g, st); break;
+    case 6: launch_gemm<L_ANY, L_K>(cl, grid, g, st); break;
+    case 7: launch_gemm<L_ANY, L_MN>(cl, grid, g, st); break;
+    default: launch_gemm<L_ANY, L_ANY>(cl, grid, g, st); break;
     }
     return hipGetLastError();
 }

@@ -50,23 +50,39 @@ def _operand(layout, R, K, nb, rng):
     return flat, (sr, sk, sb), dense
 
 
+def _result_strides(layout, M, N, ns):
+    """(cm, cn, c1, c2) of C: n unit-stride with every offset a multiple of 4
+    floats ("n"), the same along m ("m"), or odd offsets ("x")."""
+    if layout == "n":
+        cm, cn = _up4(N + 1), 1
+        c2 = M * cm + 4
+        return cm, cn, ns * c2 + 8, c2
+    if layout == "m":
+        cm, cn = 1, _up4(M + 1)
+        c2 = N * cn + 4
+        return cm, cn, ns * c2 + 8, c2
+    cm, cn = N + 3, 1
+    c2 = M * cm + 1
+    return cm, cn, ns * c2 + 7, c2
+
+
 @pytest.mark.parametrize("la", ["k", "r", "x"])
 @pytest.mark.parametrize("lb", ["k", "r", "x"])
+@pytest.mark.parametrize("lc", ["n", "m", "x"])
 @pytest.mark.parametrize("mnk", [(64, 128, 96), (36, 44, 28), (37, 45, 29), (8, 300, 4), (130, 24, 64)])
-def test_gemm_strided_layouts(la, lb, mnk):
+def test_gemm_strided_layouts(la, lb, lc, mnk):
     """Every staging layout of A (k, m or neither unit-stride) and B (k, n or
-    neither), full and ragged tiles, several k-steps, batches, alpha, beta
-    and split-K slices, against FP64 products of the bf16-rounded operands."""
+    neither), every result layout (16-byte stores along n or m, or element
+    stores), full and ragged tiles, several k-steps, batches, alpha, beta and
+    split-K slices, against FP64 products of the bf16-rounded operands."""
     M, N, K = mnk
     nb = 3
-    rng = np.random.default_rng(M * 7 + N * 3 + K + ord(la) * 11 + ord(lb))
+    rng = np.random.default_rng(M * 7 + N * 3 + K + ord(la) * 11 + ord(lb) + ord(lc) * 5)
     A, (am, ak, a1), Ad = _operand(la, M, K, nb, rng)
     Bt, (bn, bk, b1), Btd = _operand(lb, N, K, nb, rng)     # B stored as its transpose's layout: rows n
     for ksplit in (0, 32):
         ns = -(-K // ksplit) if ksplit else 1
-        cm, cn = N + 3, 1
-        c2 = M * cm + 1
-        c1 = ns * c2 + 7
+        cm, cn, c1, c2 = _result_strides(lc, M, N, ns)
         C0 = rng.standard_normal(nb * c1).astype(np.float32)
         alpha, beta = (1.0, 0.0) if ksplit else (-0.75, 0.5)
         got = G.gemm_strided(M, N, K, A, (am, ak, a1), Bt, (bk, bn, b1), C0.copy(), (cm, cn, c1, c2), nb=nb,
