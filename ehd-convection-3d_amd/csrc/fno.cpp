@@ -2,9 +2,10 @@
 //
 // Every contraction is one call of the strided bf16 matrix-core GEMM
 // (fno_kernels.hip): the truncated DFTs are dense matrices applied along one
-// axis (z: real -> complex, y and x: complex -> complex), complex products are
-// four real GEMMs over separate real and imaginary planes, and transposes or
-// conjugates are stride and sign choices.  The backward pass applies the
+// axis (z: real -> complex, y and x: complex -> complex), a complex product is
+// one launch over separate real and imaginary planes (each plane read once,
+// each result plane written once), and transposes or conjugates are stride
+// and sign choices.  The backward pass applies the
 // adjoint of each stage in reverse order (the conjugate-transposed DFT
 // matrices, the conjugated mixing weights) and forms the weight gradients as
 // GEMMs summed over the batch.
@@ -124,16 +125,40 @@ hipError_t rgemm(ehd_fno* f, const Geo& g, const float* A, const float* B, float
     return gemm_bf16(a, st);
 }
 
+// One launch of the complex product (Ar + i sa Ai)(Br + i sb Bi) (sa, sb = -1
+// conjugate): Cr = alpha Re(.) + beta Cr, Ci likewise with Im(.).  Ai or Bi
+// null: that operand is real; Ci null: only the real part is written.
+hipError_t xgemm(ehd_fno* f, const Geo& g, const float* Ar, const float* Ai, const float* Br, const float* Bi,
+                 float* Cr, float* Ci, int sa, int sb, float alpha, float beta, hipStream_t st)
+{
+    const long long kmax = g.K - 1;
+    const long long la = (g.nb1 - 1) * g.a1 + (g.nb2 - 1) * g.a2 + (g.M - 1) * g.am + kmax * g.ak;
+    const long long lb = (g.nb1 - 1) * g.b1 + (g.nb2 - 1) * g.b2 + kmax * g.bk + (g.N - 1) * g.bn;
+    const long long lc = (g.nb1 - 1) * g.c1 + (g.nb2 - 1) * g.c2 + (g.M - 1) * g.cm + (g.N - 1) * g.cn;
+    const bool neg = g.am < 0 || g.ak < 0 || g.bk < 0 || g.bn < 0 || g.cm < 0 || g.cn < 0 || g.a1 < 0 ||
+                     g.a2 < 0 || g.b1 < 0 || g.b2 < 0 || g.c1 < 0 || g.c2 < 0;
+    if (neg || g.ksplit != 0 || !in_range(f, Ar, la) || (Ai && !in_range(f, Ai, la)) || !in_range(f, Br, lb) ||
+        (Bi && !in_range(f, Bi, lb)) || !in_range(f, Cr, lc) || (Ci && !in_range(f, Ci, lc)))
+        return hipErrorInvalidValue;
+    CGemmArgs x{GemmArgs{g.M, g.N, g.K, Ar, g.am, g.ak, Br, g.bk, g.bn, Cr, g.cm, g.cn, g.nb1, g.nb2, g.a1, g.a2,
+                         g.b1, g.b2, g.c1, g.c2, alpha, beta, 0},
+                Ai, Bi, Ci, (float)sa, (float)sb};
+    // real products: Ar Br, Ai Bi (both imaginary parts), Ar Bi and Ai Br (for Ci)
+    const int nprod = 1 + (Ai && Bi ? 1 : 0) + (Ci ? (Ai ? 1 : 0) + (Bi ? 1 : 0) : 0);
+    const double nb = (double)g.nb1 * g.nb2;
+    f->work.flops += 2.0 * g.M * (double)g.N * g.K * nb * nprod;
+    f->work.bytes += 4.0 * nb * ((double)g.M * g.K * (Ai ? 2 : 1) + (double)g.K * g.N * (Bi ? 2 : 1) +
+                                 (double)g.M * g.N * (Ci ? 2 : 1) * (beta != 0.f ? 2 : 1));
+    f->work.launches += 1;
+    return cgemm_bf16(x, st);
+}
+
 // C = op(A) op(B) (+ beta C) for complex operands in separate planes; sa/sb =
 // -1 conjugates A/B:  Cr = Ar Br - sa sb Ai Bi,  Ci = sb Ar Bi + sa Ai Br.
 hipError_t cgemm(ehd_fno* f, const Geo& g, const float* Ar, const float* Ai, const float* Br, const float* Bi,
                  float* Cr, float* Ci, int sa, int sb, float beta, hipStream_t st)
 {
-    hipError_t e;
-    if ((e = rgemm(f, g, Ar, Br, Cr, 1.f, beta, st)) != hipSuccess) return e;
-    if ((e = rgemm(f, g, Ai, Bi, Cr, -(float)(sa * sb), 1.f, st)) != hipSuccess) return e;
-    if ((e = rgemm(f, g, Ar, Bi, Ci, (float)sb, beta, st)) != hipSuccess) return e;
-    return rgemm(f, g, Ai, Br, Ci, (float)sa, 1.f, st);
+    return xgemm(f, g, Ar, Ai, Br, Bi, Cr, Ci, sa, sb, 1.f, beta, st);
 }
 
 int calloc_c(ehd_fno* f, Cbuf* b, long long n)
@@ -299,8 +324,7 @@ int spectral_fwd(ehd_fno* f, int l, int B, const float* v, float* out, float bet
         g.bk = P; g.bn = 1; g.b1 = Z * P;
         g.cm = P; g.cn = 1; g.c1 = KZ * P;
         g.nb1 = (int)BC;
-        FCHK(rgemm(f, g, f->AzT.r, v, f->Vz.r, 1.f, 0.f, f->st));
-        FCHK(rgemm(f, g, f->AzT.i, v, f->Vz.i, 1.f, 0.f, f->st));
+        FCHK(xgemm(f, g, f->AzT.r, f->AzT.i, v, nullptr, f->Vz.r, f->Vz.i, 1, 1, 1.f, 0.f, f->st));
     }
     {   // 2. Vy = ByT Vz along y, per (bc, kz)
         Geo g;
@@ -352,8 +376,7 @@ int spectral_fwd(ehd_fno* f, int l, int B, const float* v, float* out, float bet
         g.bk = P; g.bn = 1; g.b1 = KZ * P;
         g.cm = P; g.cn = 1; g.c1 = Z * P;
         g.nb1 = (int)BC;
-        FCHK(rgemm(f, g, f->Azi.r, f->Wy.r, out, 1.f, beta, f->st));
-        FCHK(rgemm(f, g, f->Azi.i, f->Wy.i, out, -1.f, 1.f, f->st));
+        FCHK(xgemm(f, g, f->Azi.r, f->Azi.i, f->Wy.r, f->Wy.i, out, nullptr, 1, 1, 1.f, beta, f->st));
     }
     return EHD_OK;
 }
@@ -376,8 +399,7 @@ int spectral_bwd(ehd_fno* f, int l, int B, const float* du, float* dv, float bet
         g.bk = P; g.bn = 1; g.b1 = Z * P;
         g.cm = P; g.cn = 1; g.c1 = KZ * P;
         g.nb1 = (int)BC;
-        FCHK(rgemm(f, g, f->Azi.r, du, f->Wy.r, 1.f, 0.f, f->st));
-        FCHK(rgemm(f, g, f->Azi.i, du, f->Wy.i, -1.f, 0.f, f->st));
+        FCHK(xgemm(f, g, f->Azi.r, f->Azi.i, du, nullptr, f->Wy.r, f->Wy.i, -1, 1, 1.f, 0.f, f->st));
     }
     {   // 6b. G_Wx = conj(Byi) G_Wy along y
         Geo g;
@@ -438,8 +460,7 @@ int spectral_bwd(ehd_fno* f, int l, int B, const float* du, float* dv, float bet
         g.bk = P; g.bn = 1; g.b1 = KZ * P;
         g.cm = P; g.cn = 1; g.c1 = Z * P;
         g.nb1 = (int)BC;
-        FCHK(rgemm(f, g, f->AzT.r, f->Vz.r, dv, 1.f, beta, f->st));
-        FCHK(rgemm(f, g, f->AzT.i, f->Vz.i, dv, 1.f, 1.f, f->st));
+        FCHK(xgemm(f, g, f->AzT.r, f->AzT.i, f->Vz.r, f->Vz.i, dv, nullptr, -1, 1, 1.f, beta, f->st));
     }
     return EHD_OK;
 }
@@ -765,6 +786,51 @@ int ehd_fno_gemm_strided(int M, int N, int K, int nb, int ksplit, const float* A
     if (e == hipSuccess) e = hipMemcpy(C, dC, (size_t)c_len * 4, hipMemcpyDeviceToHost);
     cleanup();
     if (e != hipSuccess) return set_error(EHD_EHIP, "GEMM failed: %s", hipGetErrorString(e));
+    return EHD_OK;
+}
+
+int ehd_fno_cgemm_strided(int M, int N, int K, int nb, const float* Ar, const float* Ai, const long long as[3],
+                          long long a_len, const float* Br, const float* Bi, const long long bs[3], long long b_len,
+                          float* Cr, float* Ci, const long long cs[3], long long c_len, int sa, int sb, float alpha,
+                          float beta)
+{
+    if (M < 1 || N < 1 || K < 1 || nb < 1 || !Ar || !Br || !Cr || !as || !bs || !cs || (sa != 1 && sa != -1) ||
+        (sb != 1 && sb != -1))
+        return set_error(EHD_EINVAL, "bad complex GEMM arguments");
+    for (int i = 0; i < 3; ++i)
+        if (as[i] < 0 || bs[i] < 0 || cs[i] < 0) return set_error(EHD_EINVAL, "negative stride");
+    const long long la = (nb - 1) * as[2] + (M - 1) * as[0] + (long long)(K - 1) * as[1];
+    const long long lb = (nb - 1) * bs[2] + (long long)(K - 1) * bs[0] + (N - 1) * bs[1];
+    const long long lc = (nb - 1) * cs[2] + (M - 1) * cs[0] + (N - 1) * cs[1];
+    if (la >= a_len || lb >= b_len || lc >= c_len) return set_error(EHD_EINVAL, "strides reach outside the arrays");
+    // device copies of the planes given: [Ar, Ai, Br, Bi, Cr, Ci]
+    const float* src[6] = {Ar, Ai, Br, Bi, Cr, Ci};
+    const long long len[6] = {a_len, a_len, b_len, b_len, c_len, c_len};
+    float* d[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    auto cleanup = [&]() {
+        for (float* q : d)
+            if (q) (void)hipFree(q);
+    };
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 6 && e == hipSuccess; ++i) {
+        if (!src[i]) continue;
+        if (hipMalloc(&d[i], (size_t)len[i] * 4) != hipSuccess) {
+            cleanup();
+            return set_error(EHD_ENOMEM, "complex GEMM allocation failed");
+        }
+        e = hipMemcpy(d[i], src[i], (size_t)len[i] * 4, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) {
+        CGemmArgs x{GemmArgs{M, N, K, d[0], as[0], as[1], d[2], bs[0], bs[1], d[4], cs[0], cs[1], nb, 1, as[2], 0,
+                             bs[2], 0, cs[2], 0, alpha, beta, 0},
+                    d[1], d[3], d[5], (float)sa, (float)sb};
+        e = cgemm_bf16(x, nullptr);
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(Cr, d[4], (size_t)c_len * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && Ci) e = hipMemcpy(Ci, d[5], (size_t)c_len * 4, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return set_error(EHD_EHIP, "complex GEMM failed: %s", hipGetErrorString(e));
     return EHD_OK;
 }
 

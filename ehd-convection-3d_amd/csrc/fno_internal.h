@@ -23,6 +23,20 @@ struct GemmArgs {
 };
 
 hipError_t gemm_bf16(const GemmArgs& g, hipStream_t st);
+// Complex product on the matrix cores: with Ar = g.A, Br = g.B, Cr = g.C,
+//   Cr = alpha * Re[(Ar + i sa Ai)(Br + i sb Bi)] + beta * Cr,
+//   Ci = alpha * Im[(Ar + i sa Ai)(Br + i sb Bi)] + beta * Ci,
+// each imaginary plane at the same strides as its real plane; Ai or Bi null:
+// that operand is real; Ci null: only Cr is written.  sa, sb = +1 or -1
+// (-1 conjugates).  No split-K.
+struct CGemmArgs {
+    GemmArgs g;
+    const float* Ai;
+    const float* Bi;
+    float* Ci;
+    float sa, sb;
+};
+hipError_t cgemm_bf16(const CGemmArgs& x, hipStream_t st);
 // staging layout gemm_bf16 picks for A and B: 0 k unit-stride, 1 m (n) unit-stride
 // (16-byte loads), 2 element loads
 int gemm_layout_a(const GemmArgs& g);

@@ -57,6 +57,7 @@ constexpr int C_M = 2;       // m unit-stride
 constexpr int CP = TN + 4;   // C tile pitch in LDS (floats)
 constexpr int kOperandLds = 2 * TM * LP * 2 + 2 * TN * LP * 2;
 constexpr int kGemmLds = kOperandLds > TM * CP * 4 ? kOperandLds : TM * CP * 4;
+constexpr int kCgemmLds = 2 * kOperandLds > TM * CP * 4 ? 2 * kOperandLds : TM * CP * 4;
 
 // One operand tile (64 rows of m or n, 32 of k) in flight: 8 floats per thread.
 struct Stage {
@@ -132,6 +133,70 @@ __device__ __forceinline__ void store_tile(const Stage& st, unsigned short (*T)[
     }
 }
 
+// Writes one 64 x 64 tile of C = alpha * acc (+ beta * C); acc in the MFMA
+// result layout (D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4).
+// The vector layouts stage the tile in lds (TM * CP floats, which the caller
+// has retired from every other use) and leave with 16-byte stores.
+template <int CL>
+__device__ __forceinline__ void store_result(const GemmArgs& g, float* __restrict__ C, const f32x4 (&acc)[2][2],
+                                             unsigned char* lds, int m0, int n0, int wm, int wn, int tid)
+{
+    const int lane = tid & 63;
+    const int lj = lane & 15, li4 = (lane >> 4) * 4;
+    if constexpr (CL == C_ANY) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int gm = m0 + wm + a * 16 + li4 + e, gn = n0 + wn + b * 16 + lj;
+                    if (gm < g.M && gn < g.N) {
+                        float* c = C + (long long)gm * g.cm + (long long)gn * g.cn;
+                        const float v = g.alpha * acc[a][b][e];
+                        *c = (g.beta != 0.f) ? v + g.beta * *c : v;
+                    }
+                }
+    } else {
+        float(*Cs)[CP] = reinterpret_cast<float(*)[CP]>(lds);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int i = wm + a * 16 + li4 + e, j = wn + b * 16 + lj;
+                    if constexpr (CL == C_N) Cs[i][j] = acc[a][b][e];
+                    else Cs[j][i] = acc[a][b][e];
+                }
+        __syncthreads();
+        // 64 rows (m for C_N, n for C_M) of 16 quads along the unit-stride axis
+        const int r0 = CL == C_N ? m0 : n0, c0 = CL == C_N ? n0 : m0;
+        const int R = CL == C_N ? g.M : g.N, Q = CL == C_N ? g.N : g.M;
+        const long long sr = CL == C_N ? g.cm : g.cn;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int q = tid + 256 * h, r = q >> 4, c4 = 4 * (q & 15);
+            if (r0 + r < R && c0 + c4 < Q) {   // Q is a multiple of 4: the quad lies inside
+                float* c = C + (long long)(r0 + r) * sr + (c0 + c4);
+                float4 v = *(const float4*)&Cs[r][c4];
+                v.x = g.alpha * v.x;
+                v.y = g.alpha * v.y;
+                v.z = g.alpha * v.z;
+                v.w = g.alpha * v.w;
+                if (g.beta != 0.f) {
+                    const float4 o = *(const float4*)c;
+                    v.x = v.x + g.beta * o.x;
+                    v.y = v.y + g.beta * o.y;
+                    v.z = v.z + g.beta * o.z;
+                    v.w = v.w + g.beta * o.w;
+                }
+                *(float4*)c = v;
+            }
+        }
+    }
+}
+
 template <int AL, int BL, int CL>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
 {
@@ -188,60 +253,110 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
         __syncthreads();
         buf ^= 1;
     }
-    // D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4
-    const int lj = lane & 15, li4 = (lane >> 4) * 4;
-    if constexpr (CL == C_ANY) {
+    // the last k-step's barrier has retired every LDS read of the operands
+    store_result<CL>(g, C, acc, lds, m0, n0, wm, wn, tid);
+}
+
+__device__ __forceinline__ void negate(Stage& s)
+{
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+    for (int i = 0; i < 8; ++i) s.v[i] = -s.v[i];
+}
+
+// The complex product (Ar + i sa Ai)(Br + i sb Bi) on the same tiles, real
+// and imaginary parts in separate planes with shared strides: an absent
+// imaginary operand is zero, an absent Ci is not written.  The signs are
+// applied while staging (exact in bf16); three accumulator sets hold
+// Ar Br, (sa Ai)(sb Bi) and Ar (sb Bi) + (sa Ai) Br, so each output plane is
+// written once and each operand plane read once.
+template <int AL, int BL, int CL>
+__global__ void __launch_bounds__(256) cgemm_bf16_kernel(CGemmArgs x)
+{
+    const GemmArgs& g = x.g;
+    __shared__ __align__(16) unsigned char lds[kCgemmLds];
+    // [buffer][part: real, imaginary][row][k]
+    unsigned short(*As)[2][TM][LP] = reinterpret_cast<unsigned short(*)[2][TM][LP]>(lds);
+    unsigned short(*Bs)[2][TN][LP] = reinterpret_cast<unsigned short(*)[2][TN][LP]>(lds + 2 * 2 * TM * LP * 2);
+    const bool hai = x.Ai != nullptr, hbi = x.Bi != nullptr, hci = x.Ci != nullptr;
+    const bool nega = x.sa < 0.f, negb = x.sb < 0.f;
+    const int bz = blockIdx.z;
+    const int i1 = bz / g.nb2, i2 = bz - i1 * g.nb2;
+    const long long oa = i1 * g.a1 + i2 * g.a2, ob = i1 * g.b1 + i2 * g.b2, oc = i1 * g.c1 + i2 * g.c2;
+    const float* __restrict__ Ar = g.A + oa;
+    const float* __restrict__ Ai = hai ? x.Ai + oa : nullptr;
+    const float* __restrict__ Br = g.B + ob;
+    const float* __restrict__ Bi = hbi ? x.Bi + ob : nullptr;
+    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f32x4 rr[2][2], ii[2][2], ri[2][2];
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int gm = m0 + wm + a * 16 + li4 + e, gn = n0 + wn + b * 16 + lj;
-                    if (gm < g.M && gn < g.N) {
-                        float* c = C + (long long)gm * g.cm + (long long)gn * g.cn;
-                        const float v = g.alpha * acc[a][b][e];
-                        *c = (g.beta != 0.f) ? v + g.beta * *c : v;
-                    }
-                }
-    } else {
-        // the last k-step's barrier has retired every LDS read of the operands
-        float(*Cs)[CP] = reinterpret_cast<float(*)[CP]>(lds);
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int i = wm + a * 16 + li4 + e, j = wn + b * 16 + lj;
-                    if constexpr (CL == C_N) Cs[i][j] = acc[a][b][e];
-                    else Cs[j][i] = acc[a][b][e];
-                }
-        __syncthreads();
-        // 64 rows (m for C_N, n for C_M) of 16 quads along the unit-stride axis
-        const int r0 = CL == C_N ? m0 : n0, c0 = CL == C_N ? n0 : m0;
-        const int R = CL == C_N ? g.M : g.N, Q = CL == C_N ? g.N : g.M;
-        const long long sr = CL == C_N ? g.cm : g.cn;
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            const int q = tid + 256 * h, r = q >> 4, c4 = 4 * (q & 15);
-            if (r0 + r < R && c0 + c4 < Q) {   // Q is a multiple of 4: the quad lies inside
-                float* c = C + (long long)(r0 + r) * sr + (c0 + c4);
-                float4 v = *(const float4*)&Cs[r][c4];
-                v.x = g.alpha * v.x;
-                v.y = g.alpha * v.y;
-                v.z = g.alpha * v.z;
-                v.w = g.alpha * v.w;
-                if (g.beta != 0.f) {
-                    const float4 o = *(const float4*)c;
-                    v.x = v.x + g.beta * o.x;
-                    v.y = v.y + g.beta * o.y;
-                    v.z = v.z + g.beta * o.z;
-                    v.w = v.w + g.beta * o.w;
-                }
-                *(float4*)c = v;
-            }
+        for (int b = 0; b < 2; ++b) {
+            rr[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+            ii[a][b] = rr[a][b];
+            ri[a][b] = rr[a][b];
         }
+    Stage sar, sai, sbr, sbi;
+    auto load = [&](int k0) {
+        load_tile<AL>(sar, Ar, g.am, g.ak, m0, g.M, k0, g.K, tid);
+        if (hai) load_tile<AL>(sai, Ai, g.am, g.ak, m0, g.M, k0, g.K, tid);
+        load_tile<BL>(sbr, Br, g.bn, g.bk, n0, g.N, k0, g.K, tid);
+        if (hbi) load_tile<BL>(sbi, Bi, g.bn, g.bk, n0, g.N, k0, g.K, tid);
+    };
+    auto stage = [&](int buf) {
+        store_tile<AL>(sar, As[buf][0], tid, g.am, g.ak);
+        if (hai) {
+            if (nega) negate(sai);
+            store_tile<AL>(sai, As[buf][1], tid, g.am, g.ak);
+        }
+        store_tile<BL>(sbr, Bs[buf][0], tid, g.bn, g.bk);
+        if (hbi) {
+            if (negb) negate(sbi);
+            store_tile<BL>(sbi, Bs[buf][1], tid, g.bn, g.bk);
+        }
+    };
+    load(0);
+    stage(0);
+    __syncthreads();
+    const int li = lane & 15, lk = (lane >> 4) * 8;
+    int buf = 0;
+    for (int k0 = 0; k0 < g.K; k0 += TK) {
+        const bool more = k0 + TK < g.K;
+        if (more) load(k0 + TK);
+        bf16x8 ar[2], br[2], ai[2] = {}, bi[2] = {};
+#pragma unroll
+        for (int a = 0; a < 2; ++a) __builtin_memcpy(&ar[a], &As[buf][0][wm + a * 16 + li][lk], 16);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) __builtin_memcpy(&br[b], &Bs[buf][0][wn + b * 16 + li][lk], 16);
+        if (hai)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) __builtin_memcpy(&ai[a], &As[buf][1][wm + a * 16 + li][lk], 16);
+        if (hbi)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) __builtin_memcpy(&bi[b], &Bs[buf][1][wn + b * 16 + li][lk], 16);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                rr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[a], br[b], rr[a][b], 0, 0, 0);
+                if (hai && hbi) ii[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai[a], bi[b], ii[a][b], 0, 0, 0);
+                if (hci && hbi) ri[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[a], bi[b], ri[a][b], 0, 0, 0);
+                if (hci && hai) ri[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai[a], br[b], ri[a][b], 0, 0, 0);
+            }
+        if (more) stage(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) rr[a][b] -= ii[a][b];
+    store_result<CL>(g, g.C + oc, rr, lds, m0, n0, wm, wn, tid);
+    if (hci) {
+        __syncthreads();   // every lane is done with the staged real tile
+        store_result<CL>(g, x.Ci + oc, ri, lds, m0, n0, wm, wn, tid);
     }
 }
 
@@ -470,12 +585,40 @@ int gemm_layout_c(const GemmArgs& g)
 }
 
 namespace {
-template <int AL, int BL>
-void launch_gemm(int cl, dim3 grid, const GemmArgs& g, hipStream_t st)
+template <int AL, int BL, int CL>
+void launch(dim3 grid, const GemmArgs& g, hipStream_t st)
 {
-    if (cl == C_N) gemm_bf16_kernel<AL, BL, C_N><<<grid, 256, 0, st>>>(g);
-    else if (cl == C_M) gemm_bf16_kernel<AL, BL, C_M><<<grid, 256, 0, st>>>(g);
-    else gemm_bf16_kernel<AL, BL, C_ANY><<<grid, 256, 0, st>>>(g);
+    gemm_bf16_kernel<AL, BL, CL><<<grid, 256, 0, st>>>(g);
+}
+
+template <int AL, int BL, int CL>
+void launch(dim3 grid, const CGemmArgs& x, hipStream_t st)
+{
+    cgemm_bf16_kernel<AL, BL, CL><<<grid, 256, 0, st>>>(x);
+}
+
+template <int AL, int BL, class Args>
+void launch_c(int cl, dim3 grid, const Args& a, hipStream_t st)
+{
+    if (cl == C_N) launch<AL, BL, C_N>(grid, a, st);
+    else if (cl == C_M) launch<AL, BL, C_M>(grid, a, st);
+    else launch<AL, BL, C_ANY>(grid, a, st);
+}
+
+template <class Args>
+void dispatch(int al, int bl, int cl, dim3 grid, const Args& a, hipStream_t st)
+{
+    switch (al * 3 + bl) {
+    case 0: launch_c<L_K, L_K>(cl, grid, a, st); break;
+    case 1: launch_c<L_K, L_MN>(cl, grid, a, st); break;
+    case 2: launch_c<L_K, L_ANY>(cl, grid, a, st); break;
+    case 3: launch_c<L_MN, L_K>(cl, grid, a, st); break;
+    case 4: launch_c<L_MN, L_MN>(cl, grid, a, st); break;
+    case 5: launch_c<L_MN, L_ANY>(cl, grid, a, st); break;
+    case 6: launch_c<L_ANY, L_K>(cl, grid, a, st); break;
+    case 7: launch_c<L_ANY, L_MN>(cl, grid, a, st); break;
+    default: launch_c<L_ANY, L_ANY>(cl, grid, a, st); break;
+    }
 }
 }  // namespace
 
@@ -484,18 +627,26 @@ hipError_t gemm_bf16(const GemmArgs& g, hipStream_t st)
     if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.nb1 <= 0 || g.nb2 <= 0) return hipErrorInvalidValue;
     dim3 grid((g.M + TM - 1) / TM, (g.N + TN - 1) / TN, g.nb1 * g.nb2);
     if (grid.y > 65535 || grid.z > 65535) return hipErrorInvalidValue;
-    const int al = gemm_layout_a(g), bl = gemm_layout_b(g), cl = gemm_layout_c(g);
-    switch (al * 3 + bl) {
-    case 0: launch_gemm<L_K, L_K>(cl, grid, g, st); break;
-    case 1: launch_gemm<L_K, L_MN>(cl, grid, g, st); break;
-    case 2: launch_gemm<L_K, L_ANY>(cl, grid, g, st); break;
-    case 3: launch_gemm<L_MN, L_K>(cl, grid, g, st); break;
-    case 4: launch_gemm<L_MN, L_MN>(cl, grid, g, st); break;
-    case 5: launch_gemm<L_MN, L_ANY>(cl, grid, g, st); break;
-    case 6: launch_gemm<L_ANY, L_K>(cl, grid, g, st); break;
-    case 7: launch_gemm<L_ANY, L_MN>(cl, grid, g, st); break;
-    default: launch_gemm<L_ANY, L_ANY>(cl, grid, g, st); break;
-    }
+    dispatch(gemm_layout_a(g), gemm_layout_b(g), gemm_layout_c(g), grid, g, st);
+    return hipGetLastError();
+}
+
+hipError_t cgemm_bf16(const CGemmArgs& x, hipStream_t st)
+{
+    const GemmArgs& g = x.g;
+    if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.nb1 <= 0 || g.nb2 <= 0 || g.ksplit != 0) return hipErrorInvalidValue;
+    dim3 grid((g.M + TM - 1) / TM, (g.N + TN - 1) / TN, g.nb1 * g.nb2);
+    if (grid.y > 65535 || grid.z > 65535) return hipErrorInvalidValue;
+    // the two planes of an operand share its strides; a 16-byte form needs both bases aligned
+    int al = gemm_layout_a(g), bl = gemm_layout_b(g), cl = gemm_layout_c(g);
+    GemmArgs im = g;
+    im.A = x.Ai ? x.Ai : g.A;
+    im.B = x.Bi ? x.Bi : g.B;
+    im.C = x.Ci ? x.Ci : g.C;
+    if (gemm_layout_a(im) != al) al = L_ANY;
+    if (gemm_layout_b(im) != bl) bl = L_ANY;
+    if (gemm_layout_c(im) != cl) cl = C_ANY;
+    dispatch(al, bl, cl, grid, x, st);
     return hipGetLastError();
 }
 

@@ -22,7 +22,7 @@ class Work(C.Structure):
 
 EXPORTS = ("ehd_fno_create", "ehd_fno_destroy", "ehd_fno_num_params", "ehd_fno_set_params", "ehd_fno_get_params",
            "ehd_fno_forward", "ehd_fno_gradients", "ehd_fno_train_step", "ehd_fno_gemm", "ehd_fno_gemm_strided",
-           "ehd_fno_gemm_work", "ehd_fno_set_dataset", "ehd_fno_train_step_indices")
+           "ehd_fno_gemm_work", "ehd_fno_set_dataset", "ehd_fno_train_step_indices", "ehd_fno_cgemm_strided")
 
 _bound = False
 
@@ -49,12 +49,15 @@ def _lib():
         L.ehd_fno_gemm_strided.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fp, llp, C.c_longlong, fp,
                                            llp, C.c_longlong, fp, llp, C.c_longlong, C.c_float, C.c_float]
         L.ehd_fno_gemm_work.argtypes = [vp, C.c_int, C.POINTER(Work)]
+        L.ehd_fno_cgemm_strided.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, llp, C.c_longlong, fp, fp, llp,
+                                            C.c_longlong, fp, fp, llp, C.c_longlong, C.c_int, C.c_int, C.c_float,
+                                            C.c_float]
         L.ehd_fno_set_dataset.argtypes = [vp, C.c_int, fp, fp]
         L.ehd_fno_train_step_indices.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_float,
                                                  C.POINTER(C.c_double)]
         for n in ("ehd_fno_set_params", "ehd_fno_get_params", "ehd_fno_forward", "ehd_fno_gradients",
                   "ehd_fno_train_step", "ehd_fno_gemm", "ehd_fno_gemm_strided", "ehd_fno_gemm_work",
-                  "ehd_fno_set_dataset", "ehd_fno_train_step_indices"):
+                  "ehd_fno_set_dataset", "ehd_fno_train_step_indices", "ehd_fno_cgemm_strided"):
             getattr(L, n).restype = C.c_int
         _bound = True
     return L
@@ -88,6 +91,27 @@ def gemm_strided(M, N, K, A, a_strides, B, b_strides, C_, c_strides, nb=1, kspli
     _check(_lib().ehd_fno_gemm_strided(M, N, K, nb, ksplit, _fp(A), arr(a_strides), A.size, _fp(B), arr(b_strides),
                                        B.size, _fp(Cf), arr(c_strides), Cf.size, float(alpha), float(beta)))
     return C_
+
+
+def cgemm_strided(M, N, K, Ar, Ai, a_strides, Br, Bi, b_strides, Cr, Ci, c_strides, nb=1, sa=1, sb=1, alpha=1.0,
+                  beta=0.0):
+    """The library's one-launch complex product (ehd_fno_cgemm_strided) on flat
+    host planes: Cr + i Ci = alpha (Ar + i sa Ai)(Br + i sb Bi) + beta (Cr + i Ci);
+    Ai, Bi or Ci None: that plane is absent.  Strides as in gemm_strided
+    (c_strides = (m, n, batch)); Cr and Ci are updated in place."""
+    planes = []
+    for X in (Ar, Ai, Br, Bi):
+        planes.append(None if X is None else np.ascontiguousarray(X, dtype=np.float32).reshape(-1))
+    for X in (Cr, Ci):
+        if X is not None:
+            assert X.dtype == np.float32 and X.flags.c_contiguous
+    ptr = lambda X: None if X is None else _fp(X.reshape(-1))
+    arr = lambda v: (C.c_longlong * len(v))(*[int(x) for x in v])
+    _check(_lib().ehd_fno_cgemm_strided(M, N, K, nb, ptr(planes[0]), ptr(planes[1]), arr(a_strides), planes[0].size,
+                                        ptr(planes[2]), ptr(planes[3]), arr(b_strides), planes[2].size, ptr(Cr),
+                                        ptr(Ci), arr(c_strides), Cr.size, int(sa), int(sb), float(alpha),
+                                        float(beta)))
+    return Cr, Ci
 
 
 # flat parameter order of include/ehd_fno.h

@@ -86,6 +86,19 @@ int ehd_fno_gemm_strided(int M, int N, int K, int nb, int ksplit, const float* A
                          long long a_len, const float* B, const long long bs[3], long long b_len, float* C,
                          const long long cs[4], long long c_len, float alpha, float beta);
 
+/* The complex product the spectral layers use, in one launch:
+ *   Cr = alpha Re[(Ar + i sa Ai)(Br + i sb Bi)] + beta Cr,
+ *   Ci = alpha Im[(Ar + i sa Ai)(Br + i sb Bi)] + beta Ci,
+ * each plane indexed as A, B and C above (the imaginary plane of an operand
+ * at the same strides as its real plane; cs[3] unused).  sa, sb = +1 or -1
+ * (-1: conjugate).  Ai or Bi NULL: that operand is real; Ci NULL: only Cr is
+ * written.  Host arrays of a_len, b_len, c_len floats per plane (checked as
+ * above). */
+int ehd_fno_cgemm_strided(int M, int N, int K, int nb, const float* Ar, const float* Ai, const long long as[3],
+                          long long a_len, const float* Br, const float* Bi, const long long bs[3], long long b_len,
+                          float* Cr, float* Ci, const long long cs[3], long long c_len, int sa, int sb, float alpha,
+                          float beta);
+
 /* Work of the GEMMs issued since the last reset (forward, gradients and
  * training steps): algorithmic FLOPs (2 M N K per real product; a complex
  * product counts as its four real ones), algorithmic HBM bytes (A and B read

@@ -147,3 +147,28 @@ def test_strided_gemm_rejects_bad_layouts_on_host():
     rc = L.ehd_fno_gemm_strided(4, 4, 32, 1, 4, fp(A), arr((0, 1, 0)), 64, fp(B), arr((1, 0, 0)), 64, fp(Cm),
                                 arr((4, 1, 0, 16)), 64, 1.0, 0.0)
     assert rc == -1 and b"outside" in L.ehd_last_error()
+
+
+def test_complex_gemm_rejects_bad_arguments_on_host():
+    """ehd_fno_cgemm_strided checks its arguments before any device work: signs
+    other than +-1, strides that reach outside the host arrays, negative
+    strides and a missing real plane are refused with EHD_EINVAL."""
+    import ctypes as C
+    import numpy as np
+    from ehd_amd import fno as G
+
+    L = G._lib()
+    P = [np.zeros(64, np.float32) for _ in range(6)]
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    arr = lambda v: (C.c_longlong * len(v))(*v)
+    ok = (4, 1, 16)
+
+    def call(sa=1, sb=1, a_str=ok, a_len=64, ar=True):
+        return L.ehd_fno_cgemm_strided(4, 4, 4, 1, fp(P[0]) if ar else None, fp(P[1]), arr(a_str), a_len, fp(P[2]),
+                                       fp(P[3]), arr(ok), 64, fp(P[4]), fp(P[5]), arr(ok), 64, sa, sb, 1.0, 0.0)
+
+    assert call(sa=2) == -1 and b"complex GEMM" in L.ehd_last_error()
+    assert call(sb=0) == -1
+    assert call(a_str=(5, 1, 0), a_len=16) == -1 and b"outside" in L.ehd_last_error()
+    assert call(a_str=(4, -1, 0)) == -1 and b"negative" in L.ehd_last_error()
+    assert call(ar=False) == -1

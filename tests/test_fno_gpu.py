@@ -107,6 +107,62 @@ def test_gemm_strided_layouts(la, lb, lc, mnk):
         assert np.array_equal(got[mask], C0[mask])
 
 
+# (imaginary A, imaginary B, imaginary C written, sa, sb): the forms the
+# spectral layers use (complex x complex, complex x real, real part only)
+# and the rest
+CMODES = [(True, True, True, 1, 1), (True, True, True, -1, 1), (True, True, True, 1, -1),
+          (True, False, True, -1, 1), (True, True, False, -1, 1), (False, True, True, 1, -1),
+          (False, False, True, 1, 1)]
+
+
+@pytest.mark.parametrize("la", ["k", "r", "x"])
+@pytest.mark.parametrize("lb", ["k", "r", "x"])
+@pytest.mark.parametrize("lc", ["n", "m", "x"])
+@pytest.mark.parametrize("mnk", [(36, 44, 40), (37, 45, 29)])
+def test_cgemm_strided_forms(la, lb, lc, mnk):
+    """The one-launch complex product in every staging and result layout,
+    each form of it (an absent imaginary plane, the real part only,
+    conjugates), alpha and beta, against FP64 complex products of the
+    bf16-rounded planes; elements outside C are untouched."""
+    M, N, K = mnk
+    nb = 2
+    rng = np.random.default_rng(M * 5 + N + K * 3 + ord(la) * 7 + ord(lb) * 13 + ord(lc) * 17)
+    Ar, (am, ak, a1), Ard = _operand(la, M, K, nb, rng)
+    Br, (bn, bk, b1), Brd = _operand(lb, N, K, nb, rng)
+    Ai, Bi = rng.standard_normal(Ar.size).astype(np.float32), rng.standard_normal(Br.size).astype(np.float32)
+    r, k = np.meshgrid(np.arange(M), np.arange(K), indexing="ij")
+    Aid = np.stack([Ai[b * a1 + r * am + k * ak] for b in range(nb)])
+    r, k = np.meshgrid(np.arange(N), np.arange(K), indexing="ij")
+    Bid = np.stack([Bi[b * b1 + r * bn + k * bk] for b in range(nb)])
+    cm, cn, c1, _ = _result_strides(lc, M, N, 1)
+    m, n = np.meshgrid(np.arange(M), np.arange(N), indexing="ij")
+    for hai, hbi, hci, sa, sb in CMODES:
+        alpha, beta = (0.75, -0.5) if hci else (1.0, 0.0)
+        C0r = rng.standard_normal(nb * c1).astype(np.float32)
+        C0i = rng.standard_normal(nb * c1).astype(np.float32)
+        gr, gi = G.cgemm_strided(M, N, K, Ar, Ai if hai else None, (am, ak, a1), Br, Bi if hbi else None,
+                                 (bk, bn, b1), C0r.copy(), C0i.copy() if hci else None, (cm, cn, c1), nb=nb, sa=sa,
+                                 sb=sb, alpha=alpha, beta=beta)
+        mask = np.ones(C0r.size, bool)
+        for b in range(nb):
+            Ab = _bf16(Ard[b]) + 1j * sa * (_bf16(Aid[b]) if hai else 0)
+            Bb = (_bf16(Brd[b]) + 1j * sb * (_bf16(Bid[b]) if hbi else 0)).T
+            idx = b * c1 + m * cm + n * cn
+            mask[idx.ravel()] = False
+            prod = Ab.astype(np.complex128) @ Bb.astype(np.complex128)
+            scale = np.abs(Ab) @ np.abs(Bb) + abs(beta) * (np.abs(C0r[idx]) + np.abs(C0i[idx]))
+            ref_r = alpha * prod.real + beta * C0r[idx]
+            err = np.abs(gr[idx] - ref_r)
+            assert np.all(err <= 1e-5 * scale + 1e-6), ("re", b, hai, hbi, hci, sa, sb, float(np.max(err / scale)))
+            if hci:
+                ref_i = alpha * prod.imag + beta * C0i[idx]
+                err = np.abs(gi[idx] - ref_i)
+                assert np.all(err <= 1e-5 * scale + 1e-6), ("im", b, hai, hbi, sa, sb, float(np.max(err / scale)))
+        assert np.array_equal(gr[mask], C0r[mask])
+        if hci:
+            assert np.array_equal(gi[mask], C0i[mask])
+
+
 CFG = F.FNOConfig(X=16, Y=12, Z=8, width=8, modes=(4, 3, 3), layers=3, d_in=6, d_hidden=16, d_out=5)
 
 
