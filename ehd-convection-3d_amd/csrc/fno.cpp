@@ -105,8 +105,18 @@ bool in_range(const ehd_fno* f, const float* base, long long last)
     return false;
 }
 
+// Elementwise tail of a GEMM's epilogue: + bias[m], then GELU (the value
+// before it kept in pre) or the product with gelu'(dact); pre and dact at
+// C's offsets.
+struct Epi {
+    const float* bias = nullptr;
+    int act = kActNone;
+    float* pre = nullptr;
+    const float* dact = nullptr;
+};
+
 hipError_t rgemm(ehd_fno* f, const Geo& g, const float* A, const float* B, float* C, float alpha, float beta,
-                 hipStream_t st)
+                 hipStream_t st, const Epi& ep = Epi{})
 {
     const long long kmax = g.K - 1;
     const long long la = (g.nb1 - 1) * g.a1 + (g.nb2 - 1) * g.a2 + (g.M - 1) * g.am + kmax * g.ak;
@@ -115,12 +125,20 @@ hipError_t rgemm(ehd_fno* f, const Geo& g, const float* A, const float* B, float
     const bool neg = g.am < 0 || g.ak < 0 || g.bk < 0 || g.bn < 0 || g.cm < 0 || g.cn < 0 || g.a1 < 0 ||
                      g.a2 < 0 || g.b1 < 0 || g.b2 < 0 || g.c1 < 0 || g.c2 < 0;
     if (neg || !in_range(f, A, la) || !in_range(f, B, lb) || !in_range(f, C, lc)) return hipErrorInvalidValue;
+    if ((ep.bias && !in_range(f, ep.bias, g.M - 1)) || (ep.act == kActGelu && !in_range(f, ep.pre, lc)) ||
+        (ep.act == kActGeluGrad && !in_range(f, ep.dact, lc)) || (ep.act != kActNone && g.ksplit > 0))
+        return hipErrorInvalidValue;
     GemmArgs a{g.M, g.N, g.K, A, g.am, g.ak, B, g.bk, g.bn, C, g.cm, g.cn,
                g.nb1, g.nb2, g.a1, g.a2, g.b1, g.b2, g.c1, g.c2, alpha, beta, g.ksplit};
+    a.bias = ep.bias;
+    a.act = ep.act;
+    a.pre = ep.pre;
+    a.dact = ep.dact;
     // with split-K the nb2 slices share the K products of one batch entry
     const double nb = (double)g.nb1 * (g.ksplit > 0 ? 1 : g.nb2), nc = (double)g.nb1 * g.nb2;
     f->work.flops += 2.0 * g.M * (double)g.N * g.K * nb;
-    f->work.bytes += 4.0 * (((double)g.M * g.K + (double)g.K * g.N) * nb + (double)g.M * g.N * nc * (beta != 0.f ? 2 : 1));
+    f->work.bytes += 4.0 * (((double)g.M * g.K + (double)g.K * g.N) * nb +
+                            (double)g.M * g.N * nc * ((beta != 0.f ? 2 : 1) + (ep.act != kActNone ? 1 : 0)));
     f->work.launches += 1;
     return gemm_bf16(a, st);
 }
@@ -264,8 +282,10 @@ int make_dft(ehd_fno* f)
 
 // ---- stages --------------------------------------------------------------
 
-// C[b][o][s] (+)= sum_c Wt[c][o] X[b][c][s]  (pointwise layer, weight [cin][cout])
-int pointwise(ehd_fno* f, int B, int cin, int cout, const float* Wt, const float* X, float* C, float beta)
+// C[b][o][s] (+)= sum_c Wt[c][o] X[b][c][s]  (pointwise layer, weight [cin][cout]),
+// then the epilogue's bias and activation
+int pointwise(ehd_fno* f, int B, int cin, int cout, const float* Wt, const float* X, float* C, float beta,
+              const Epi& ep)
 {
     Geo g;
     g.M = cout; g.N = (int)f->ZP; g.K = cin;
@@ -273,12 +293,13 @@ int pointwise(ehd_fno* f, int B, int cin, int cout, const float* Wt, const float
     g.bk = f->ZP; g.bn = 1; g.b1 = (long long)cin * f->ZP;
     g.cm = f->ZP; g.cn = 1; g.c1 = (long long)cout * f->ZP;
     g.nb1 = B;
-    FCHK(rgemm(f, g, Wt, X, C, 1.f, beta, f->st));
+    FCHK(rgemm(f, g, Wt, X, C, 1.f, beta, f->st, ep));
     return EHD_OK;
 }
 
-// dX[b][c][s] (+)= sum_o Wt[c][o] dC[b][o][s]
-int pointwise_dx(ehd_fno* f, int B, int cin, int cout, const float* Wt, const float* dC, float* dX, float beta)
+// dX[b][c][s] (+)= sum_o Wt[c][o] dC[b][o][s], then (u given) times gelu'(u)
+int pointwise_dx(ehd_fno* f, int B, int cin, int cout, const float* Wt, const float* dC, float* dX, float beta,
+                 const float* u = nullptr)
 {
     Geo g;
     g.M = cin; g.N = (int)f->ZP; g.K = cout;
@@ -286,7 +307,12 @@ int pointwise_dx(ehd_fno* f, int B, int cin, int cout, const float* Wt, const fl
     g.bk = f->ZP; g.bn = 1; g.b1 = (long long)cout * f->ZP;
     g.cm = f->ZP; g.cn = 1; g.c1 = (long long)cin * f->ZP;
     g.nb1 = B;
-    FCHK(rgemm(f, g, Wt, dC, dX, 1.f, beta, f->st));
+    Epi ep;
+    if (u) {
+        ep.act = kActGeluGrad;
+        ep.dact = u;
+    }
+    FCHK(rgemm(f, g, Wt, dC, dX, 1.f, beta, f->st, ep));
     return EHD_OK;
 }
 
@@ -469,17 +495,28 @@ int forward(ehd_fno* f, int B)
 {
     const int w = f->c.width, L = f->c.layers;
     int rc;
-    if ((rc = pointwise(f, B, f->c.d_in, w, f->p + f->oPw, f->in, f->v[0], 0.f))) return rc;
-    FCHK(bias_act(f->v[0], nullptr, f->p + f->oPb, B, w, f->ZP, false, f->st));
+    // each layer's bias and GELU run in the epilogue of the GEMM that completes its sum
+    Epi ep;
+    ep.bias = f->p + f->oPb;
+    if ((rc = pointwise(f, B, f->c.d_in, w, f->p + f->oPw, f->in, f->v[0], 0.f, ep))) return rc;
     for (int l = 0; l < L; ++l) {
         if ((rc = spectral_fwd(f, l, B, f->v[l], f->v[l + 1], 0.f))) return rc;
-        if ((rc = pointwise(f, B, w, w, f->p + f->oW[l], f->v[l], f->v[l + 1], 1.f))) return rc;
-        FCHK(bias_act(f->v[l + 1], l < L - 1 ? f->u[l] : nullptr, f->p + f->ob[l], B, w, f->ZP, l < L - 1, f->st));
+        Epi el;
+        el.bias = f->p + f->ob[l];
+        if (l < L - 1) {
+            el.act = kActGelu;
+            el.pre = f->u[l];
+        }
+        if ((rc = pointwise(f, B, w, w, f->p + f->oW[l], f->v[l], f->v[l + 1], 1.f, el))) return rc;
     }
-    if ((rc = pointwise(f, B, w, f->c.d_hidden, f->p + f->oQ1w, f->v[L], f->h, 0.f))) return rc;
-    FCHK(bias_act(f->h, f->z1, f->p + f->oQ1b, B, f->c.d_hidden, f->ZP, true, f->st));
-    if ((rc = pointwise(f, B, f->c.d_hidden, f->c.d_out, f->p + f->oQ2w, f->h, f->out, 0.f))) return rc;
-    FCHK(bias_act(f->out, nullptr, f->p + f->oQ2b, B, f->c.d_out, f->ZP, false, f->st));
+    Epi eq1;
+    eq1.bias = f->p + f->oQ1b;
+    eq1.act = kActGelu;
+    eq1.pre = f->z1;
+    if ((rc = pointwise(f, B, w, f->c.d_hidden, f->p + f->oQ1w, f->v[L], f->h, 0.f, eq1))) return rc;
+    Epi eq2;
+    eq2.bias = f->p + f->oQ2b;
+    if ((rc = pointwise(f, B, f->c.d_hidden, f->c.d_out, f->p + f->oQ2w, f->h, f->out, 0.f, eq2))) return rc;
     return EHD_OK;
 }
 
@@ -491,19 +528,19 @@ int backward(ehd_fno* f, int B)
     int rc;
     if ((rc = pointwise_dw(f, B, H, O, f->h, f->dout, f->g + f->oQ2w))) return rc;
     FCHK(channel_sum(f->dout, f->g + f->oQ2b, B, O, ZP, false, f->red, f->st));
-    if ((rc = pointwise_dx(f, B, H, O, f->p + f->oQ2w, f->dout, f->dh, 0.f))) return rc;
-    FCHK(gelu_bwd(f->dh, f->z1, (long long)B * H * ZP, f->st));
+    if ((rc = pointwise_dx(f, B, H, O, f->p + f->oQ2w, f->dout, f->dh, 0.f, f->z1))) return rc;
     if ((rc = pointwise_dw(f, B, w, H, f->v[L], f->dh, f->g + f->oQ1w))) return rc;
     FCHK(channel_sum(f->dh, f->g + f->oQ1b, B, H, ZP, false, f->red, f->st));
     float* dv = f->dv0;
     float* dn = f->dv1;
     if ((rc = pointwise_dx(f, B, w, H, f->p + f->oQ1w, f->dh, dv, 0.f))) return rc;
+    // dv enters iteration l as the gradient of layer l's pre-activation: the
+    // input gradient of layer l + 1 leaves its GEMM times gelu'(u[l])
     for (int l = L - 1; l >= 0; --l) {
-        if (l < L - 1) FCHK(gelu_bwd(dv, f->u[l], (long long)B * w * ZP, f->st));
         if ((rc = pointwise_dw(f, B, w, w, f->v[l], dv, f->g + f->oW[l]))) return rc;
         FCHK(channel_sum(dv, f->g + f->ob[l], B, w, ZP, false, f->red, f->st));
         if ((rc = spectral_bwd(f, l, B, dv, dn, 0.f))) return rc;
-        if ((rc = pointwise_dx(f, B, w, w, f->p + f->oW[l], dv, dn, 1.f))) return rc;
+        if ((rc = pointwise_dx(f, B, w, w, f->p + f->oW[l], dv, dn, 1.f, l >= 1 ? f->u[l - 1] : nullptr))) return rc;
         float* t = dv;
         dv = dn;
         dn = t;

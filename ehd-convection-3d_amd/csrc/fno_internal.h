@@ -20,7 +20,15 @@ struct GemmArgs {
     long long a1, a2, b1, b2, c1, c2;
     float alpha, beta;
     int ksplit;        // > 0: batch index i2 takes k in [i2*ksplit, (i2+1)*ksplit) (split-K partial sums)
+    // epilogue after alpha/beta: + bias[m] (null: none), then act:
+    //   kActGelu: pre (C's offsets) = the value, C = gelu(value);
+    //   kActGeluGrad: C = value * gelu'(dact) (dact at C's offsets)
+    const float* bias = nullptr;
+    int act = 0;
+    float* pre = nullptr;
+    const float* dact = nullptr;
 };
+constexpr int kActNone = 0, kActGelu = 1, kActGeluGrad = 2;
 
 hipError_t gemm_bf16(const GemmArgs& g, hipStream_t st);
 // Complex product on the matrix cores: with Ar = g.A, Br = g.B, Cr = g.C,
@@ -48,10 +56,6 @@ int gemm_layout_c(const GemmArgs& g);
 // sum_slices_part_floats(n) floats of workspace
 long long sum_slices_part_floats(long long n);
 hipError_t sum_slices(const float* src, float* dst, long long n, int nslices, float beta, float* part, hipStream_t st);
-// y = x + bias[c] over [B][C][S]; with gelu: u keeps the pre-activation, y = gelu(u)
-hipError_t bias_act(float* y, float* u, const float* bias, int B, int C, long long S, bool gelu, hipStream_t st);
-// g *= gelu'(u)
-hipError_t gelu_bwd(float* g, const float* u, long long n, hipStream_t st);
 // db[c] (+)= sum_{b,s} g[b][c][s]; part holds channel_sum_part_floats(B, C, S) floats of workspace
 long long channel_sum_part_floats(int B, int C, long long S);
 hipError_t channel_sum(const float* g, float* db, int B, int C, long long S, bool accumulate, float* part,

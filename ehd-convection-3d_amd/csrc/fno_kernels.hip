@@ -3,7 +3,7 @@
 // that carries every contraction of the Fourier neural operator (the
 // truncated DFTs written as matrices, the per-mode channel mixing, the
 // pointwise layers and all their adjoints), plus the elementwise kernels
-// (bias, GELU, loss, column sums, Adam).
+// (loss, column sums, Adam); bias and GELU run in the GEMM epilogue.
 //
 // Operands are FP32 in memory and rounded to bf16 (nearest even) when staged
 // into LDS; accumulation and the epilogue are FP32.  Any element strides are
@@ -31,6 +31,14 @@ __device__ __forceinline__ uint32_t to_bf16(float f)
 
 __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) { return to_bf16(lo) | (to_bf16(hi) << 16); }
 
+// GELU with the exact (erf) form, and its derivative
+__device__ __forceinline__ float gelu(float v) { return 0.5f * v * (1.0f + erff(v * 0.70710678118654752f)); }
+
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * expf(-0.5f * x * x) * 0.3989422804014327f;
+}
+
 // Tile 64 x 64 x 32; 256 threads = 4 waves, each owning a 32 x 32 block of C
 // (2 x 2 MFMA blocks of 16 x 16).  Both operands sit in LDS as bf16 rows of
 // k (A as [m][k], B as [n][k]), so every lane fetches its 8 k-values of an
@@ -41,7 +49,6 @@ __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) { return to_bf
 constexpr int TM = 64, TN = 64, TK = 32, LP = TK + 8;
 constexpr int kSliceGroups = 32;     // first-pass groups of slices in sum_slices
 constexpr int kRowChunk = 8192;      // row elements per workgroup in channel_sum
-constexpr int kActIter = 4;          // float4s per thread in bias_act
 
 // operand staging layouts
 constexpr int L_K = 0;       // k unit-stride: 16-byte loads along k
@@ -133,13 +140,53 @@ __device__ __forceinline__ void store_tile(const Stage& st, unsigned short (*T)[
     }
 }
 
-// Writes one 64 x 64 tile of C = alpha * acc (+ beta * C); acc in the MFMA
-// result layout (D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4).
-// The vector layouts stage the tile in lds (TM * CP floats, which the caller
-// has retired from every other use) and leave with 16-byte stores.
-template <int CL>
-__device__ __forceinline__ void store_result(const GemmArgs& g, float* __restrict__ C, const f32x4 (&acc)[2][2],
-                                             unsigned char* lds, int m0, int n0, int wm, int wn, int tid)
+// The epilogue's elementwise tail on a result v at offset o of C (row m):
+// + bias[m], then GELU (the value before it kept in pre) or the product with
+// gelu'(dact).
+__device__ __forceinline__ float finish(const GemmArgs& g, float v, int m, long long o)
+{
+    if (g.bias) v = v + g.bias[m];
+    if (g.act == kActGelu) {
+        g.pre[o] = v;
+        v = gelu(v);
+    } else if (g.act == kActGeluGrad) {
+        v = v * gelu_grad(g.dact[o]);
+    }
+    return v;
+}
+
+// The same on a quad of C at offset o (16-byte aligned, as pre and dact
+// there): bias bm, bm + dm, bm + 2 dm, bm + 3 dm.
+__device__ __forceinline__ float4 finish4(const GemmArgs& g, float4 v, int m, int dm, long long o)
+{
+    if (g.bias) {
+        v.x = v.x + g.bias[m];
+        v.y = v.y + g.bias[m + dm];
+        v.z = v.z + g.bias[m + 2 * dm];
+        v.w = v.w + g.bias[m + 3 * dm];
+    }
+    if (g.act == kActGelu) {
+        *(float4*)(g.pre + o) = v;
+        v = make_float4(gelu(v.x), gelu(v.y), gelu(v.z), gelu(v.w));
+    } else if (g.act == kActGeluGrad) {
+        const float4 d = *(const float4*)(g.dact + o);
+        v.x = v.x * gelu_grad(d.x);
+        v.y = v.y * gelu_grad(d.y);
+        v.z = v.z * gelu_grad(d.z);
+        v.w = v.w * gelu_grad(d.w);
+    }
+    return v;
+}
+
+// Writes one 64 x 64 tile of C = alpha * acc (+ beta * C) at batch offset oc;
+// acc in the MFMA result layout (D[i][j] of a 16x16 block: lane j +
+// 16*(i/4), element i%4).  The vector layouts stage the tile in lds (TM * CP
+// floats, which the caller has retired from every other use) and leave with
+// 16-byte stores.  TAIL: then the bias and activation of finish().
+template <int CL, bool TAIL>
+__device__ __forceinline__ void store_result(const GemmArgs& g, float* __restrict__ C, long long oc,
+                                             const f32x4 (&acc)[2][2], unsigned char* lds, int m0, int n0, int wm,
+                                             int wn, int tid)
 {
     const int lane = tid & 63;
     const int lj = lane & 15, li4 = (lane >> 4) * 4;
@@ -152,9 +199,11 @@ __device__ __forceinline__ void store_result(const GemmArgs& g, float* __restric
                 for (int e = 0; e < 4; ++e) {
                     const int gm = m0 + wm + a * 16 + li4 + e, gn = n0 + wn + b * 16 + lj;
                     if (gm < g.M && gn < g.N) {
-                        float* c = C + (long long)gm * g.cm + (long long)gn * g.cn;
-                        const float v = g.alpha * acc[a][b][e];
-                        *c = (g.beta != 0.f) ? v + g.beta * *c : v;
+                        const long long o = oc + (long long)gm * g.cm + (long long)gn * g.cn;
+                        float v = g.alpha * acc[a][b][e];
+                        if (g.beta != 0.f) v = v + g.beta * C[o];
+                        if constexpr (TAIL) v = finish(g, v, gm, o);
+                        C[o] = v;
                     }
                 }
     } else {
@@ -178,20 +227,22 @@ __device__ __forceinline__ void store_result(const GemmArgs& g, float* __restric
         for (int h = 0; h < 4; ++h) {
             const int q = tid + 256 * h, r = q >> 4, c4 = 4 * (q & 15);
             if (r0 + r < R && c0 + c4 < Q) {   // Q is a multiple of 4: the quad lies inside
-                float* c = C + (long long)(r0 + r) * sr + (c0 + c4);
+                const long long o = oc + (long long)(r0 + r) * sr + (c0 + c4);
                 float4 v = *(const float4*)&Cs[r][c4];
                 v.x = g.alpha * v.x;
                 v.y = g.alpha * v.y;
                 v.z = g.alpha * v.z;
                 v.w = g.alpha * v.w;
                 if (g.beta != 0.f) {
-                    const float4 o = *(const float4*)c;
-                    v.x = v.x + g.beta * o.x;
-                    v.y = v.y + g.beta * o.y;
-                    v.z = v.z + g.beta * o.z;
-                    v.w = v.w + g.beta * o.w;
+                    const float4 w = *(const float4*)(C + o);
+                    v.x = v.x + g.beta * w.x;
+                    v.y = v.y + g.beta * w.y;
+                    v.z = v.z + g.beta * w.z;
+                    v.w = v.w + g.beta * w.w;
                 }
-                *(float4*)c = v;
+                // row m (C_N) or four consecutive m (C_M)
+                if constexpr (TAIL) v = finish4(g, v, CL == C_N ? r0 + r : c0 + c4, CL == C_N ? 0 : 1, o);
+                *(float4*)(C + o) = v;
             }
         }
     }
@@ -208,7 +259,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
     const int i1 = bz / g.nb2, i2 = bz - i1 * g.nb2;
     const float* __restrict__ A = g.A + i1 * g.a1 + i2 * g.a2;
     const float* __restrict__ B = g.B + i1 * g.b1 + i2 * g.b2;
-    float* __restrict__ C = g.C + i1 * g.c1 + i2 * g.c2;
+    const long long oc = i1 * g.c1 + i2 * g.c2;
     const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -254,7 +305,8 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
         buf ^= 1;
     }
     // the last k-step's barrier has retired every LDS read of the operands
-    store_result<CL>(g, C, acc, lds, m0, n0, wm, wn, tid);
+    if (g.bias || g.act != kActNone) store_result<CL, true>(g, g.C, oc, acc, lds, m0, n0, wm, wn, tid);
+    else store_result<CL, false>(g, g.C, oc, acc, lds, m0, n0, wm, wn, tid);
 }
 
 __device__ __forceinline__ void negate(Stage& s)
@@ -353,10 +405,10 @@ __global__ void __launch_bounds__(256) cgemm_bf16_kernel(CGemmArgs x)
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) rr[a][b] -= ii[a][b];
-    store_result<CL>(g, g.C + oc, rr, lds, m0, n0, wm, wn, tid);
+    store_result<CL, false>(g, g.C, oc, rr, lds, m0, n0, wm, wn, tid);
     if (hci) {
         __syncthreads();   // every lane is done with the staged real tile
-        store_result<CL>(g, x.Ci + oc, ri, lds, m0, n0, wm, wn, tid);
+        store_result<CL, false>(g, x.Ci, oc, ri, lds, m0, n0, wm, wn, tid);
     }
 }
 
@@ -442,71 +494,6 @@ __global__ void __launch_bounds__(256) channel_total_kernel(const float* __restr
     if (threadIdx.x == 0) db[c] = accumulate ? db[c] + tot : tot;
 }
 
-__device__ __forceinline__ float gelu(float v) { return 0.5f * v * (1.0f + erff(v * 0.70710678118654752f)); }
-
-__device__ __forceinline__ float gelu_grad(float x)
-{
-    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * expf(-0.5f * x * x) * 0.3989422804014327f;
-}
-
-// y[row][s] = x[row][s] + bias[row % C]; act: u = that, y = gelu(u).  One row
-// (b, c) per blockIdx.y, so the bias is one value per workgroup; each thread
-// takes kActIter float4s (VEC) or elements of the row.
-template <bool VEC>
-__global__ void __launch_bounds__(256) bias_act_kernel(float* __restrict__ y, float* __restrict__ u,
-                                                       const float* __restrict__ bias, int C, long long S, int act)
-{
-    const long long row = blockIdx.y;
-    const float bv = bias[row % C];
-    float* __restrict__ yr = y + row * S;
-    float* __restrict__ ur = act ? u + row * S : nullptr;
-    if constexpr (VEC) {
-        const long long base = (long long)blockIdx.x * (1024 * kActIter) + 4 * threadIdx.x;
-#pragma unroll
-        for (int r = 0; r < kActIter; ++r) {
-            const long long i = base + 1024 * r;
-            if (i >= S) break;
-            float4 v = *(float4*)(yr + i);
-            v.x += bv; v.y += bv; v.z += bv; v.w += bv;
-            if (act) {
-                *(float4*)(ur + i) = v;
-                v = make_float4(gelu(v.x), gelu(v.y), gelu(v.z), gelu(v.w));
-            }
-            *(float4*)(yr + i) = v;
-        }
-    } else {
-        const long long base = (long long)blockIdx.x * (1024 * kActIter) + threadIdx.x;
-        for (int r = 0; r < 4 * kActIter; ++r) {
-            const long long i = base + 256 * r;
-            if (i >= S) break;
-            const float v = yr[i] + bv;
-            if (act) {
-                ur[i] = v;
-                yr[i] = gelu(v);
-            } else {
-                yr[i] = v;
-            }
-        }
-    }
-}
-
-// g[i] *= gelu'(u[i]); VEC: float4 i in [0, n/4)
-template <bool VEC>
-__global__ void __launch_bounds__(256) gelu_bwd_kernel(float* __restrict__ g, const float* __restrict__ u, long long n)
-{
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    if constexpr (VEC) {
-        for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n / 4; i += stride) {
-            const float4 x = ((const float4*)u)[i];
-            float4 v = ((float4*)g)[i];
-            v.x *= gelu_grad(x.x); v.y *= gelu_grad(x.y); v.z *= gelu_grad(x.z); v.w *= gelu_grad(x.w);
-            ((float4*)g)[i] = v;
-        }
-    } else {
-        for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) g[i] *= gelu_grad(u[i]);
-    }
-}
-
 // d = out - target; grad = 2 d / n; partial sums of d^2 per workgroup
 __global__ void __launch_bounds__(256) mse_kernel(const float* __restrict__ out, const float* __restrict__ tgt,
                                                   float* __restrict__ grad, double* __restrict__ part, long long n)
@@ -575,10 +562,12 @@ int gemm_layout_b(const GemmArgs& g)
 
 // Layout of C: 16-byte stores need the unit-stride axis in whole quads
 // (extent a multiple of 4) and every other offset a multiple of 4 floats
-// from a 16-byte aligned base.
+// from a 16-byte aligned base (and so do the epilogue's pre and dact, which
+// share C's offsets).
 int gemm_layout_c(const GemmArgs& g)
 {
-    const bool base = aligned16(g.C) && mul4(g.c1) && mul4(g.c2);
+    const bool base = aligned16(g.C) && mul4(g.c1) && mul4(g.c2) && (g.act != kActGelu || aligned16(g.pre)) &&
+                      (g.act != kActGeluGrad || aligned16(g.dact));
     if (base && g.cn == 1 && mul4(g.cm) && mul4(g.N)) return C_N;
     if (base && g.cm == 1 && mul4(g.cn) && mul4(g.M)) return C_M;
     return C_ANY;
@@ -659,28 +648,6 @@ hipError_t sum_slices(const float* src, float* dst, long long n, int nslices, fl
     if (nb > 2147483647LL) return hipErrorInvalidValue;
     slice_group_sum_kernel<<<dim3((unsigned)nb, kSliceGroups), 256, 0, st>>>(src, part, n, nslices);
     group_total_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(part, dst, n, beta);
-    return hipGetLastError();
-}
-
-hipError_t bias_act(float* y, float* u, const float* bias, int B, int C, long long S, bool gelu, hipStream_t st)
-{
-    const long long rows = (long long)B * C;
-    if (rows <= 0 || rows > 65535 || S <= 0) return hipErrorInvalidValue;
-    const bool vec = (S % 4 == 0) && ((uintptr_t)y % 16 == 0) && (!gelu || (uintptr_t)u % 16 == 0);
-    const dim3 grid((unsigned)((S + 1024 * kActIter - 1) / (1024 * kActIter)), (unsigned)rows);
-    if (vec)
-        bias_act_kernel<true><<<grid, 256, 0, st>>>(y, u, bias, C, S, gelu ? 1 : 0);
-    else
-        bias_act_kernel<false><<<grid, 256, 0, st>>>(y, u, bias, C, S, gelu ? 1 : 0);
-    return hipGetLastError();
-}
-
-hipError_t gelu_bwd(float* g, const float* u, long long n, hipStream_t st)
-{
-    if (n % 4 == 0 && (uintptr_t)g % 16 == 0 && (uintptr_t)u % 16 == 0)
-        gelu_bwd_kernel<true><<<grid_of(n / 4), 256, 0, st>>>(g, u, n);
-    else
-        gelu_bwd_kernel<false><<<grid_of(n), 256, 0, st>>>(g, u, n);
     return hipGetLastError();
 }
 
