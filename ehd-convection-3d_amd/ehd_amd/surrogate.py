@@ -311,7 +311,7 @@ def main(argv=None):
                     help="training and test range of T (steady convection on this grid)")
     ap.add_argument("--C-range", type=float, nargs=2, default=(5.0, 20.0))
     ap.add_argument("--min-steps", type=int, default=0)
-    ap.add_argument("--epochs", type=int, default=400)
+    ap.add_argument("--epochs", type=int, default=1000)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--width", type=int, default=32)
     ap.add_argument("--modes", type=int, nargs=3, default=(8, 8, 16), help="Fourier modes along x, y, z")
