@@ -172,3 +172,21 @@ def test_complex_gemm_rejects_bad_arguments_on_host():
     assert call(a_str=(5, 1, 0), a_len=16) == -1 and b"outside" in L.ehd_last_error()
     assert call(a_str=(4, -1, 0)) == -1 and b"negative" in L.ehd_last_error()
     assert call(ar=False) == -1
+
+
+def test_resident_dataset_entry_points_reject_bad_arguments_on_host():
+    """ehd_fno_set_dataset and ehd_fno_train_step_indices refuse a missing
+    network, an empty set and missing arrays before any device work."""
+    import ctypes as C
+    import numpy as np
+    from ehd_amd import fno as G
+
+    L = G._lib()
+    a = np.zeros(8, np.float32)
+    fp = a.ctypes.data_as(C.POINTER(C.c_float))
+    assert L.ehd_fno_set_dataset(None, 1, fp, fp) == -1
+    assert L.ehd_fno_set_dataset(None, 0, fp, fp) == -1
+    idx = (C.c_int * 2)(0, 1)
+    loss = C.c_double()
+    assert L.ehd_fno_train_step_indices(None, 2, idx, 1e-3, C.byref(loss)) == -1
+    assert L.ehd_last_error()
