@@ -13,8 +13,9 @@ cold (hydrostatic start + noise) and warm (the network's fields through the
 consistent start, whose 5000 iterations count as steps).  Asserted: the
 acceptance rule of the surrogate (every warm run ends within 0.5 % of its
 cold run's Nu), the prediction's distance to the steady state, and the gain
-in steps to steady state over the four draws (measured 2.09x with these
-seeds; the 64-draw evaluation with 85 training states reaches 3.1x).
+in steps to steady state over the four draws (measured 2.11x with these
+seeds; the 64-draw evaluation with 85 training states and 1000 epochs
+reaches 3.61x).
 
 Why not 64x64x32 and T up to 800 as the configuration names: on dz = 1/31
 every box from T = 170 to 400 either does not convect or goes non-finite
